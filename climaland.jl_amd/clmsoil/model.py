@@ -1,0 +1,464 @@
+"""Host-side mirror of ClimaLand's soil model interface over libclmsoil.so.
+
+Names, argument meaning and call order follow the reference (ClimaLand.jl
+v1.11.2) so code written against it reads the same:
+
+  vanGenuchten(α=, n=)                      retention_models.jl:30-44
+  RichardsParameters(ν=, hydrology_cm=, K_sat=, S_s=, θ_r=)   rre.jl:7-48
+  EnergyHydrologyParameters(...)            energy_hydrology.jl:21-207
+  WaterFluxBC, FreeDrainage, HeatFluxBC, WaterHeatBC, EnergyWaterFreeDrainage
+                                            boundary_conditions.jl:45-60,452,568,588
+  RichardsModel(parameters=, domain=, boundary_conditions=)   rre.jl:92-109
+  EnergyHydrology(parameters=, domain=, boundary_conditions=) energy_hydrology.jl:251-271
+  make_update_cache, make_exp_tendency, make_compute_imp_tendency,
+  make_compute_jacobian, make_update_implicit_cache   models.jl:187-287; rre.jl; energy_hydrology.jl
+  initialize_jacobian(model) -> jacobian with .ldiv(dx, R)  implicit_timestepping.jl:63-172
+  LandSimulation(model, Y0, Δt, ...).step() / .solve()       Simulations.jl:115-332
+
+Prognostic state Y is a dict of NumPy arrays in CSR layout (level 0 = the
+bottom cell of a column): vartheta_l, intF_water and, for EnergyHydrology,
+theta_i, rho_e_int, intF_energy.  The cache p lives on the device inside the
+model's context; `model.aux()` copies it out.  Every computation runs in the
+HIP library; errors come back as ClmError (the reference raises).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _lib as L
+from .inputs import KAPPA_ICE, KAPPA_LIQ, kappa_solid, KAPPA_OM, KAPPA_QUARTZ, KAPPA_MINERALS
+
+# ---- parameters -------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class vanGenuchten:  # noqa: N801 (reference name)
+    α: np.ndarray
+    n: np.ndarray
+
+    @property
+    def m(self):
+        return 1.0 - 1.0 / np.asarray(self.n)
+
+    @property
+    def S_c(self):
+        n = np.asarray(self.n)
+        return (1.0 + ((n - 1.0) / n) ** (1.0 - 2.0 * n)) ** (-self.m)
+
+
+@dataclasses.dataclass
+class RichardsParameters:
+    ν: np.ndarray
+    hydrology_cm: vanGenuchten
+    K_sat: np.ndarray
+    S_s: np.ndarray
+    θ_r: np.ndarray
+
+
+@dataclasses.dataclass
+class EnergyHydrologyParameters:
+    ν: np.ndarray
+    hydrology_cm: vanGenuchten
+    K_sat: np.ndarray
+    S_s: np.ndarray
+    θ_r: np.ndarray
+    ν_ss_om: np.ndarray
+    ν_ss_quartz: np.ndarray
+    ν_ss_gravel: np.ndarray
+    ρc_ds: np.ndarray
+    κ_dry: np.ndarray
+    κ_sat_frozen: np.ndarray
+    κ_sat_unfrozen: np.ndarray
+
+    @classmethod
+    def derived(cls, *, ν, hydrology_cm, K_sat, S_s, θ_r, ν_ss_om, ν_ss_quartz, ν_ss_gravel,
+                ρc_ds, κ_dry):
+        """κ_sat_{frozen,unfrozen} from the composition (energy_hydrology.jl:168-172)."""
+        ks = kappa_solid(np.asarray(ν_ss_om), np.asarray(ν_ss_quartz))
+        ν = np.asarray(ν)
+        return cls(ν, hydrology_cm, K_sat, S_s, θ_r, ν_ss_om, ν_ss_quartz, ν_ss_gravel, ρc_ds,
+                   κ_dry, ks ** (1.0 - ν) * KAPPA_ICE ** ν, ks ** (1.0 - ν) * KAPPA_LIQ ** ν)
+
+
+# ---- domain and boundary conditions ------------------------------------------
+
+
+@dataclasses.dataclass
+class ColumnBatchDomain:
+    """Many independent Columns (Domains.jl:137-247), CSR-packed."""
+    col_ptr: np.ndarray  # int32 [ncol+1]
+    z_face: np.ndarray   # float64 [ncell + ncol], bottom -> top per column
+
+    @property
+    def ncol(self):
+        return self.col_ptr.size - 1
+
+    @property
+    def ncell(self):
+        return int(self.col_ptr[-1])
+
+
+@dataclasses.dataclass
+class WaterFluxBC:
+    """Prescribed water flux bc(p, t) per column (upward positive)."""
+    bc: np.ndarray
+
+
+@dataclasses.dataclass
+class HeatFluxBC:
+    bc: np.ndarray
+
+
+class FreeDrainage:
+    pass
+
+
+class EnergyWaterFreeDrainage:
+    pass
+
+
+@dataclasses.dataclass
+class WaterHeatBC:
+    water: WaterFluxBC
+    heat: HeatFluxBC
+
+
+# ---- models ------------------------------------------------------------------
+
+
+class _SoilModel:
+    model_id = -1
+
+    def __init__(self, *, parameters, domain: ColumnBatchDomain, boundary_conditions: dict,
+                 sources: tuple = (), jac_precision: int = 64, newton_max_iters: int = 3,
+                 newton_tol: float = 0.0):
+        self.parameters = parameters
+        self.domain = domain
+        self.boundary_conditions = boundary_conditions
+        self.sources = sources
+        self._lib = L.load()
+        self._ctx = C.c_void_p()
+        self._col_ptr = np.ascontiguousarray(domain.col_ptr, dtype=np.int32)
+        self._z_face = np.ascontiguousarray(domain.z_face, dtype=np.float64)
+        self._params = np.ascontiguousarray(self._param_rows(), dtype=np.float64)
+        cfg = L.Config(self.model_id, self._bottom_kind(), domain.ncol,
+                       L.iptr(self._col_ptr), L.dptr(self._z_face), L.dptr(self._params),
+                       self._params.shape[0], jac_precision, newton_max_iters, newton_tol)
+        L.check(self._lib, None, self._lib.clm_create(C.byref(self._ctx), C.byref(cfg), None))
+        self.jac_precision = jac_precision
+        self.set_forcing()
+
+    # -- helpers
+    def _check(self, code):
+        L.check(self._lib, self._ctx, code)
+
+    def close(self):
+        if self._ctx:
+            self._lib.clm_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def is_eh(self):
+        return self.model_id == L.MODEL_ENERGY_HYDROLOGY
+
+    def new_state(self) -> dict:
+        d = self.domain
+        st = {"vartheta_l": np.zeros(d.ncell), "intF_water": np.zeros(d.ncol)}
+        if self.is_eh:
+            st.update(theta_i=np.zeros(d.ncell), rho_e_int=np.zeros(d.ncell),
+                      intF_energy=np.zeros(d.ncol))
+        return st
+
+    def _state(self, Y: dict) -> L.State:
+        return L.State(*(L.dptr(Y.get(k)) for k in ("vartheta_l", "theta_i", "rho_e_int",
+                                                    "intF_water", "intF_energy")))
+
+    def _top_arrays(self):
+        top = self.boundary_conditions["top"]
+        if isinstance(top, WaterHeatBC):
+            return np.asarray(top.water.bc, float), np.asarray(top.heat.bc, float)
+        return np.asarray(top.bc, float), None
+
+    def _bottom_kind(self):
+        b = self.boundary_conditions["bottom"]
+        return L.BC_FREE_DRAINAGE if isinstance(b, (FreeDrainage, EnergyWaterFreeDrainage)) \
+            else L.BC_FLUX
+
+    def set_forcing(self):
+        """Push the current bc(p, t) values of the boundary conditions."""
+        tw, th = self._top_arrays()
+        n = self.domain.ncol
+        tw = np.ascontiguousarray(np.broadcast_to(tw, (n,)), dtype=np.float64)
+        th = None if th is None else np.ascontiguousarray(np.broadcast_to(th, (n,)), float)
+        bw = bh = None
+        b = self.boundary_conditions["bottom"]
+        if isinstance(b, WaterHeatBC):
+            bw = np.ascontiguousarray(np.broadcast_to(b.water.bc, (n,)), float)
+            bh = np.ascontiguousarray(np.broadcast_to(b.heat.bc, (n,)), float)
+        elif isinstance(b, WaterFluxBC):
+            bw = np.ascontiguousarray(np.broadcast_to(b.bc, (n,)), float)
+        self._check(self._lib.clm_set_forcing(self._ctx, L.dptr(tw), L.dptr(th), L.dptr(bw),
+                                              L.dptr(bh)))
+
+    def aux(self) -> dict:
+        """Copy of the cache p.soil (K, ψ, [θ_l, T, κ], boundary fluxes)."""
+        d = self.domain
+        a = {"K": np.zeros(d.ncell), "psi": np.zeros(d.ncell), "top_water": np.zeros(d.ncol),
+             "bottom_water": np.zeros(d.ncol)}
+        if self.is_eh:
+            a.update(theta_l=np.zeros(d.ncell), T=np.zeros(d.ncell), kappa=np.zeros(d.ncell),
+                     top_heat=np.zeros(d.ncol), bottom_heat=np.zeros(d.ncol))
+        s = L.Aux(*(L.dptr(a.get(k)) for k in ("K", "psi", "theta_l", "T", "kappa", "top_water",
+                                               "top_heat", "bottom_water", "bottom_heat")))
+        self._check(self._lib.clm_get_aux(self._ctx, C.byref(s)))
+        return a
+
+    def get_jacobian(self):
+        n = 3 * self.domain.ncell
+        ww = np.zeros(n)
+        ee = np.zeros(n) if self.is_eh else None
+        ew = np.zeros(n) if self.is_eh else None
+        self._check(self._lib.clm_get_jacobian(self._ctx, L.dptr(ww), L.dptr(ee), L.dptr(ew)))
+        return ww, ee, ew
+
+
+class RichardsModel(_SoilModel):
+    model_id = L.MODEL_RICHARDS
+
+    def _param_rows(self):
+        p = self.parameters
+        n = self.domain.ncol
+        rows = [p.ν, p.θ_r, p.hydrology_cm.α, p.hydrology_cm.n, p.K_sat, p.S_s]
+        return np.stack([np.broadcast_to(np.asarray(r, float), (n,)) for r in rows])
+
+
+class EnergyHydrology(_SoilModel):
+    model_id = L.MODEL_ENERGY_HYDROLOGY
+
+    def _param_rows(self):
+        p = self.parameters
+        n = self.domain.ncol
+        rows = [p.ν, p.θ_r, p.hydrology_cm.α, p.hydrology_cm.n, p.K_sat, p.S_s, p.ν_ss_om,
+                p.ν_ss_quartz, p.ν_ss_gravel, p.κ_dry, p.κ_sat_frozen, p.κ_sat_unfrozen,
+                p.ρc_ds]
+        return np.stack([np.broadcast_to(np.asarray(r, float), (n,)) for r in rows])
+
+
+# ---- hook factories (models.jl) ----------------------------------------------
+
+
+def make_update_cache(model: _SoilModel):
+    def update_cache(p, Y, t):
+        model._check(model._lib.clm_update_cache(model._ctx, C.byref(model._state(Y)), t))
+    return update_cache
+
+
+def make_update_implicit_cache(model: _SoilModel):
+    def update_implicit_cache(p, Y, t):
+        model._check(model._lib.clm_update_implicit_cache(model._ctx,
+                                                          C.byref(model._state(Y)), t))
+    return update_implicit_cache
+
+
+def make_exp_tendency(model: _SoilModel):
+    """update_cache! + compute_exp_tendency! (models.jl:279-287)."""
+    def exp_tendency(dY, Y, p, t):
+        model._check(model._lib.clm_exp_tendency(model._ctx, C.byref(model._state(Y)), t,
+                                                 C.byref(model._state(dY))))
+    return exp_tendency
+
+
+def make_compute_imp_tendency(model: _SoilModel):
+    def compute_imp_tendency(dY, Y, p, t):
+        model._check(model._lib.clm_imp_tendency(model._ctx, C.byref(model._state(Y)), t,
+                                                 C.byref(model._state(dY))))
+    return compute_imp_tendency
+
+
+class HIPJacobian:
+    """jac_prototype: the block matrix lives on the device; ldiv solves with it."""
+
+    def __init__(self, model: _SoilModel):
+        self.model = model
+
+    def ldiv(self, dx: dict, R: dict):
+        m = self.model
+        m._check(m._lib.clm_solve(m._ctx, C.byref(m._state(dx)), C.byref(m._state(R))))
+        return dx
+
+
+def initialize_jacobian(model: _SoilModel) -> HIPJacobian:
+    return HIPJacobian(model)
+
+
+def make_compute_jacobian(model: _SoilModel):
+    def compute_jacobian(jacobian: HIPJacobian, Y, p, dtγ, t):
+        model._check(model._lib.clm_jacobian(model._ctx, C.byref(model._state(Y)), dtγ, t))
+    return compute_jacobian
+
+
+# ---- simulation driver (Simulations.jl) ----------------------------------------
+
+
+class LandSimulation:
+    """ARS111 + NewtonsMethod stepping of a soil model.
+
+    `step()` / `solve()` run whole steps on the device (clm_step / clm_run).
+    `step_hooks()` drives the same step through the fine-grained hooks in
+    the ClimaTimeSteppers order, which is what a ccall drop-in under the
+    reference's own integrator executes.
+    """
+
+    def __init__(self, model: _SoilModel, Y0: dict, Δt: float, t0: float = 0.0,
+                 max_iters: int = 3, tol: float = 0.0):
+        self.model = model
+        self.Y = {k: np.array(v, dtype=np.float64, copy=True) for k, v in Y0.items()}
+        self.Δt = float(Δt)
+        self.t = float(t0)
+        self.max_iters = max_iters
+        self.tol = tol
+        self._on_device = False
+
+    def _push(self):
+        if not self._on_device:
+            m = self.model
+            m._check(m._lib.clm_set_state(m._ctx, C.byref(m._state(self.Y))))
+            self._on_device = True
+
+    def _pull(self):
+        if self._on_device:
+            m = self.model
+            m._check(m._lib.clm_get_state(m._ctx, C.byref(m._state(self.Y))))
+            self._on_device = False
+
+    def state(self) -> dict:
+        self._pull()
+        return self.Y
+
+    def step(self) -> L.StepStats:
+        m = self.model
+        self._push()
+        st = L.StepStats()
+        m._check(m._lib.clm_step(m._ctx, self.t, self.Δt, C.byref(st)))
+        self.t += self.Δt
+        return st
+
+    def set_forcing_table(self, top_water, top_heat=None):
+        """Upload a [slot][col] forcing schedule (DriverUpdateCallback values)."""
+        m = self.model
+        tw = np.ascontiguousarray(top_water, dtype=np.float64)
+        th = None if top_heat is None else np.ascontiguousarray(top_heat, dtype=np.float64)
+        m._check(m._lib.clm_set_forcing_table(m._ctx, tw.shape[0], L.dptr(tw), L.dptr(th)))
+
+    def reset(self, Y0: dict, t0: float = 0.0):
+        """Replace the state (host copy; pushed to the device on the next step)."""
+        self.Y = {k: np.array(v, dtype=np.float64, copy=True) for k, v in Y0.items()}
+        self._on_device = False
+        self.t = float(t0)
+
+    def solve(self, top_water, top_heat, nsteps: int, steps_per_slot: int,
+              want_iters: bool = False, upload: bool = True):
+        """nsteps steps with a [slot][col] forcing table, as one graph replay."""
+        m = self.model
+        if upload:
+            self.set_forcing_table(top_water, top_heat)
+        self._push()
+        st = L.StepStats()
+        it = np.zeros((nsteps, m.domain.ncol), dtype=np.int32) if want_iters else None
+        m._check(m._lib.clm_run(m._ctx, self.t, self.Δt, nsteps, steps_per_slot, C.byref(st),
+                                L.iptr(it) if it is not None else None))
+        self.t += nsteps * self.Δt
+        return st, it
+
+    def step_hooks(self):
+        """One ARS111 step through the fine-grained hooks (CTS call order)."""
+        self._pull()
+        m = self.model
+        exp_tendency = make_exp_tendency(m)
+        imp_tendency = make_compute_imp_tendency(m)
+        jacobian = make_compute_jacobian(m)
+        cache_imp = make_update_implicit_cache(m)
+        W = initialize_jacobian(m)
+        dt, t = self.Δt, self.t
+        u = self.Y
+        Texp = m.new_state()
+        exp_tendency(Texp, u, None, t)
+        U0 = {k: u[k] + dt * Texp[k] for k in u}
+        x = {k: v.copy() for k, v in U0.items()}
+        Timp, R, dx = m.new_state(), m.new_state(), m.new_state()
+        iters = self.max_iters
+        for k in range(1, self.max_iters + 1):
+            jacobian(W, x, None, dt, t + dt)
+            imp_tendency(Timp, x, None, t + dt)
+            for key in x:
+                R[key][:] = U0[key] + dt * Timp[key] - x[key]
+            W.ldiv(dx, R)
+            for key in x:
+                x[key] -= dx[key]
+            conv = False
+            if self.tol > 0:
+                err = _column_norm(m, dx, x)
+                conv = bool(np.all(err < self.tol))
+            cache_imp(None, x, t + dt)
+            if conv:
+                iters = k
+                break
+        self.Y = x
+        self.t += dt
+        return iters
+
+
+def _column_norm(m: _SoilModel, dx: dict, x: dict) -> np.ndarray:
+    """Per-column max|Δx| over the cell unknowns, ρe_int scaled by 1/ρc_s
+    with the lagged θ_l and the iterate's θ_i."""
+    from .inputs import CP_I, CP_L, RHO_I, RHO_L
+    d = np.abs(dx["vartheta_l"])
+    if m.is_eh:
+        a = m.aux()
+        rc = np.repeat(np.asarray(m.parameters.ρc_ds, float), np.diff(m.domain.col_ptr))
+        rcs = rc + a["theta_l"] * (CP_L * RHO_L) + x["theta_i"] * (CP_I * RHO_I)
+        d = np.maximum(np.maximum(d, np.abs(dx["theta_i"])), np.abs(dx["rho_e_int"]) / rcs)
+    return np.maximum.reduceat(d, m.domain.col_ptr[:-1])
+
+
+def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | None = None,
+                     tol: float | None = None, slot: int = 0):
+    """Build the model a synthetic ColumnBatch describes."""
+    w = batch.workload
+    P = batch.params
+    hcm = vanGenuchten(P[L.P_VG_ALPHA], P[L.P_VG_N])
+    dom = ColumnBatchDomain(batch.col_ptr, batch.z_face)
+    kw = dict(jac_precision=jac_precision or w.jac_precision,
+              newton_max_iters=max_iters or w.newton_max_iters,
+              newton_tol=w.newton_tol if tol is None else tol)
+    if batch.model == L.MODEL_RICHARDS:
+        par = RichardsParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R])
+        bcs = {"top": WaterFluxBC(batch.top_water[slot]), "bottom": FreeDrainage()}
+        return RichardsModel(parameters=par, domain=dom, boundary_conditions=bcs, **kw)
+    par = EnergyHydrologyParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R],
+                                    P[L.P_NU_SS_OM], P[L.P_NU_SS_QUARTZ], P[L.P_NU_SS_GRAVEL],
+                                    P[L.P_RHOC_DS], P[L.P_KAPPA_DRY], P[L.P_KAPPA_SAT_FROZEN],
+                                    P[L.P_KAPPA_SAT_UNFROZEN])
+    bcs = {"top": WaterHeatBC(WaterFluxBC(batch.top_water[slot]),
+                              HeatFluxBC(batch.top_heat[slot])),
+           "bottom": EnergyWaterFreeDrainage()}
+    return EnergyHydrology(parameters=par, domain=dom, boundary_conditions=bcs, **kw)
+
+
+__all__ = [
+    "vanGenuchten", "RichardsParameters", "EnergyHydrologyParameters", "ColumnBatchDomain",
+    "WaterFluxBC", "HeatFluxBC", "FreeDrainage", "EnergyWaterFreeDrainage", "WaterHeatBC",
+    "RichardsModel", "EnergyHydrology", "make_update_cache", "make_update_implicit_cache",
+    "make_exp_tendency", "make_compute_imp_tendency", "make_compute_jacobian",
+    "initialize_jacobian", "LandSimulation", "model_from_batch",
+    "KAPPA_OM", "KAPPA_QUARTZ", "KAPPA_MINERALS",
+]

@@ -1,0 +1,1006 @@
+// clm_soil.hip -- libclmsoil.so: C ABI (include/clm_soil.h) over the gfx950
+// soil-column kernels in clm_rre.h / clm_eh.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/clm_soil.h"
+#include "clm_eh.h"
+#include "clm_layout.h"
+#include "clm_rre.h"
+
+using namespace clm;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevStats {
+    unsigned long long sum_iters, nonconv, nan, domain, nrec;
+    unsigned int max_iters;
+    unsigned int pad;
+};
+
+}  // namespace
+
+struct clm_ctx {
+    int model = 0, bc_bottom = 0, jac_prec = 64, max_iters = 3;
+    double tol = 0.0;
+    int64_t ncol = 0, ncell = 0;
+    int nparam = 0;
+    std::vector<int32_t> col_ptr;
+    // layout
+    int32_t npack = 0;
+    int64_t nslot = 0, npcell = 0, npface = 0;
+    std::vector<int32_t> slot_col, slot_nl;
+    std::vector<int64_t> cell_off, face_off;
+    int64_t* d_cell_off = nullptr;
+    int64_t* d_face_off = nullptr;
+    int32_t* d_nl = nullptr;
+    int32_t* d_slot_col = nullptr;
+    double* d_par = nullptr;
+    double* d_zf = nullptr;
+    // state
+    double *d_xv = nullptr, *d_xi = nullptr, *d_xe = nullptr, *d_iw = nullptr, *d_ie = nullptr;
+    // forcing
+    double *d_ftw = nullptr, *d_fth = nullptr, *d_fbw = nullptr, *d_fbh = nullptr;
+    double *d_tab_w = nullptr, *d_tab_h = nullptr;
+    int32_t tab_slots = 0;
+    // aux (fine hooks)
+    double *d_K = nullptr, *d_psi = nullptr, *d_thl = nullptr, *d_T = nullptr, *d_kap = nullptr;
+    double *d_tw = nullptr, *d_th = nullptr, *d_bw = nullptr, *d_bh = nullptr;
+    bool have_cache = false, have_jac = false;
+    // fine-hook work buffers
+    double* d_wy[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double* d_wd[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* d_W = nullptr;  // 9 planes of JT (ww lo/di/up, ee, ew)
+    int32_t* d_bad = nullptr;
+    // fused-step buffers
+    double *d_U0v = nullptr, *d_U0e = nullptr, *d_Kl = nullptr, *d_kl = nullptr, *d_tl = nullptr;
+    double *d_Tp = nullptr, *d_Pp = nullptr, *d_Re = nullptr;
+    void* d_scr = nullptr;  // 6 planes of JT
+    int32_t* d_iters = nullptr;
+    int64_t iters_cap = 0;  // in steps
+    DevStats* d_stats = nullptr;
+    // stream, graph, events
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    int32_t g_nsteps = -1, g_sps = -1, g_tab = -1;
+    double g_dt = 0.0;
+    double last_ms = 0.0;
+    int32_t last_launches = 0;
+    std::string err;
+};
+
+// ---------------------------------------------------------------------------
+namespace {
+
+int set_err(clm_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c)
+        c->err = buf;
+    else
+        g_create_error = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return set_err((ctx), CLM_E_HIP, "%s failed: %s (%s:%d)", #call,              \
+                           hipGetErrorString(e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+template <typename T>
+int dalloc(clm_ctx* c, T** p, int64_t n) {
+    size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess)
+        return set_err(c, CLM_E_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess)
+        return set_err(c, CLM_E_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    return CLM_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+bool is_eh(const clm_ctx* c) { return c->model == CLM_MODEL_ENERGY_HYDROLOGY; }
+size_t jt_size(const clm_ctx* c) { return c->jac_prec == 32 ? sizeof(float) : sizeof(double); }
+
+Layout layout(const clm_ctx* c) {
+    Layout L;
+    L.cell_off = c->d_cell_off;
+    L.face_off = c->d_face_off;
+    L.nl = c->d_nl;
+    L.slot_col = c->d_slot_col;
+    L.npack = c->npack;
+    L.nslot = c->nslot;
+    return L;
+}
+
+dim3 grid_slots(const clm_ctx* c) { return dim3((unsigned)((c->nslot + 255) / 256)); }
+
+// host CSR <-> packed
+void pack_cells_host(const clm_ctx* c, const double* csr, std::vector<double>& out) {
+    out.assign((size_t)c->npcell, 0.0);
+    for (int64_t s = 0; s < c->nslot; ++s) {
+        int32_t col = c->slot_col[s];
+        if (col < 0) continue;
+        int64_t base = c->cell_off[s >> 6] + (s & 63);
+        const double* src = csr + c->col_ptr[col];
+        for (int32_t j = 0; j < c->slot_nl[s]; ++j) out[base + 64 * j] = src[j];
+    }
+}
+
+void unpack_cells_host(const clm_ctx* c, const std::vector<double>& in, double* csr) {
+    for (int64_t s = 0; s < c->nslot; ++s) {
+        int32_t col = c->slot_col[s];
+        if (col < 0) continue;
+        int64_t base = c->cell_off[s >> 6] + (s & 63);
+        double* dst = csr + c->col_ptr[col];
+        for (int32_t j = 0; j < c->slot_nl[s]; ++j) dst[j] = in[base + 64 * j];
+    }
+}
+
+void pack_cols_host(const clm_ctx* c, const double* col, std::vector<double>& out) {
+    out.assign((size_t)c->nslot, 0.0);
+    if (!col) return;
+    for (int64_t s = 0; s < c->nslot; ++s)
+        if (c->slot_col[s] >= 0) out[s] = col[c->slot_col[s]];
+}
+
+void unpack_cols_host(const clm_ctx* c, const std::vector<double>& in, double* col) {
+    for (int64_t s = 0; s < c->nslot; ++s)
+        if (c->slot_col[s] >= 0) col[c->slot_col[s]] = in[s];
+}
+
+int upload_cells(clm_ctx* c, const double* csr, double* d) {
+    if (!csr) return set_err(c, CLM_E_ARG, "missing cell array");
+    std::vector<double> h;
+    pack_cells_host(c, csr, h);
+    HIPCHK(c, hipMemcpyAsync(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int upload_cols(clm_ctx* c, const double* col, double* d) {
+    std::vector<double> h;
+    pack_cols_host(c, col, h);
+    HIPCHK(c, hipMemcpyAsync(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int download_cells(clm_ctx* c, const double* d, double* csr) {
+    if (!csr) return CLM_OK;
+    std::vector<double> h((size_t)c->npcell);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpack_cells_host(c, h, csr);
+    return CLM_OK;
+}
+
+int download_cols(clm_ctx* c, const double* d, double* col) {
+    if (!col) return CLM_OK;
+    std::vector<double> h((size_t)c->nslot);
+    HIPCHK(c, hipMemcpyAsync(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unpack_cols_host(c, h, col);
+    return CLM_OK;
+}
+
+// upload a host state into the fine-hook work buffers
+int upload_state(clm_ctx* c, const clm_state* Y, double* const* dst) {
+    if (!Y || !Y->vartheta_l) return set_err(c, CLM_E_ARG, "state: vartheta_l required");
+    int r;
+    if ((r = upload_cells(c, Y->vartheta_l, dst[0]))) return r;
+    if (is_eh(c)) {
+        if (!Y->theta_i || !Y->rho_e_int)
+            return set_err(c, CLM_E_ARG, "EnergyHydrology state needs theta_i and rho_e_int");
+        if ((r = upload_cells(c, Y->theta_i, dst[1]))) return r;
+        if ((r = upload_cells(c, Y->rho_e_int, dst[2]))) return r;
+        if ((r = upload_cols(c, Y->intF_energy, dst[4]))) return r;
+    }
+    return upload_cols(c, Y->intF_water, dst[3]);
+}
+
+int download_state(clm_ctx* c, double* const* src, clm_state* Y) {
+    int r;
+    if ((r = download_cells(c, src[0], Y->vartheta_l))) return r;
+    if ((r = download_cols(c, src[3], Y->intF_water))) return r;
+    if (is_eh(c)) {
+        if ((r = download_cells(c, src[1], Y->theta_i))) return r;
+        if ((r = download_cells(c, src[2], Y->rho_e_int))) return r;
+        if ((r = download_cols(c, src[4], Y->intF_energy))) return r;
+    }
+    return CLM_OK;
+}
+
+__global__ void k_zero_stats(DevStats* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        s->sum_iters = s->nonconv = s->nan = s->domain = s->nrec = 0;
+        s->max_iters = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_stats(const int32_t* __restrict__ it,
+                                                      const int32_t* __restrict__ nl,
+                                                      int64_t nslot, int64_t nrec,
+                                                      DevStats* __restrict__ st) {
+    unsigned long long sum = 0, nc = 0, nn = 0, nd = 0, cnt = 0;
+    unsigned int mx = 0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrec;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        if (nl[r % nslot] <= 0) continue;
+        int32_t v = it[r];
+        unsigned int k = (unsigned int)(v & 0xffff);
+        sum += k;
+        mx = k > mx ? k : mx;
+        nc += (v & 0x10000) ? 1 : 0;
+        nn += (v & 0x20000) ? 1 : 0;
+        nd += (v & 0x40000) ? 1 : 0;
+        cnt += 1;
+    }
+    // wavefront reduction then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_down(sum, off);
+        nc += __shfl_down(nc, off);
+        nn += __shfl_down(nn, off);
+        nd += __shfl_down(nd, off);
+        cnt += __shfl_down(cnt, off);
+        unsigned int o = __shfl_down(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&st->sum_iters, sum);
+        atomicAdd(&st->nonconv, nc);
+        atomicAdd(&st->nan, nn);
+        atomicAdd(&st->domain, nd);
+        atomicAdd(&st->nrec, cnt);
+        atomicMax(&st->max_iters, mx);
+    }
+}
+
+int collect_stats(clm_ctx* c, int64_t nsteps, clm_step_stats* out) {
+    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, c->d_stats);
+    int64_t nrec = nsteps * c->nslot;
+    unsigned g = (unsigned)std::min<int64_t>((nrec + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(g), dim3(256), 0, c->stream, c->d_iters, c->d_nl,
+                       c->nslot, nrec, c->d_stats);
+    HIPCHK(c, hipGetLastError());
+    DevStats h;
+    HIPCHK(c, hipMemcpyAsync(&h, c->d_stats, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->max_iters_used = (int32_t)h.max_iters;
+    out->pad_ = 0;
+    out->mean_iters = h.nrec ? (double)h.sum_iters / (double)h.nrec : 0.0;
+    out->n_nonconverged = (int64_t)h.nonconv;
+    out->n_nan = (int64_t)h.nan;
+    out->n_domain_error = (int64_t)h.domain;
+    return CLM_OK;
+}
+
+int ensure_iters(clm_ctx* c, int64_t nsteps) {
+    if (nsteps <= c->iters_cap) return CLM_OK;
+    dfree(c->d_iters);
+    int r = dalloc(c, &c->d_iters, nsteps * c->nslot);
+    if (r) return r;
+    c->iters_cap = nsteps;
+    return CLM_OK;
+}
+
+template <typename JT>
+EHScratch<JT> eh_scratch(clm_ctx* c) {
+    JT* b = (JT*)c->d_scr;
+    int64_t n = c->npcell;
+    return EHScratch<JT>{b, b + n, b + 2 * n, b + 3 * n, b + 4 * n, b + 5 * n};
+}
+
+EHBufs eh_bufs(clm_ctx* c) {
+    return EHBufs{c->d_xv, c->d_xi, c->d_xe, c->d_U0v, c->d_U0e, c->d_Kl, c->d_kl,
+                  c->d_tl, c->d_Tp, c->d_Pp, c->d_Re, c->d_iw, c->d_ie};
+}
+
+// launch one fused step on the ctx stream (capturable: kernels only)
+template <typename JT>
+void launch_step_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
+                   int32_t* iters) {
+    Layout L = layout(c);
+    dim3 g = grid_slots(c), b(256);
+    if (!is_eh(c)) {
+        JT* cp = (JT*)c->d_scr;
+        JT* dp = cp + c->npcell;
+        hipLaunchKernelGGL(k_rre_step<JT>, g, b, 0, c->stream, L, c->d_par, c->d_zf, c->d_xv,
+                           c->d_U0v, c->d_iw, ftw, c->d_fbw, c->bc_bottom, dt, c->max_iters,
+                           c->tol, cp, dp, iters);
+    } else {
+        hipLaunchKernelGGL(k_eh_step<JT>, g, b, 0, c->stream, L, c->d_par, c->d_zf,
+                           eh_bufs(c), eh_scratch<JT>(c), ftw, fth, c->d_fbw, c->d_fbh,
+                           c->bc_bottom, dt, c->max_iters, c->tol, iters);
+    }
+}
+
+void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters) {
+    if (c->jac_prec == 32)
+        launch_step_t<float>(c, ftw, fth, dt, iters);
+    else
+        launch_step_t<double>(c, ftw, fth, dt, iters);
+}
+
+void destroy_graph(clm_ctx* c) {
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->g_nsteps = -1;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int clm_abi_version(void) { return CLM_ABI_VERSION; }
+
+const char* clm_last_error(const clm_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
+    if (!out || !cfg) return set_err(nullptr, CLM_E_ARG, "clm_create: null argument");
+    *out = nullptr;
+    if (cfg->model != CLM_MODEL_RICHARDS && cfg->model != CLM_MODEL_ENERGY_HYDROLOGY)
+        return set_err(nullptr, CLM_E_ARG, "unknown model %d", cfg->model);
+    if (cfg->bc_bottom != CLM_BC_FLUX && cfg->bc_bottom != CLM_BC_FREE_DRAINAGE)
+        return set_err(nullptr, CLM_E_ARG, "unknown bottom boundary %d", cfg->bc_bottom);
+    int need = cfg->model == CLM_MODEL_RICHARDS ? CLM_NPARAM_RRE : CLM_NPARAM_EH;
+    if (cfg->nparam < need)
+        return set_err(nullptr, CLM_E_ARG, "nparam %d < %d required", cfg->nparam, need);
+    if (cfg->ncol <= 0 || cfg->ncol > (int64_t)1 << 30 || !cfg->col_ptr || !cfg->z_face ||
+        !cfg->params)
+        return set_err(nullptr, CLM_E_ARG, "bad column batch (ncol=%lld)", (long long)cfg->ncol);
+    if (cfg->jac_precision != 64 && cfg->jac_precision != 32)
+        return set_err(nullptr, CLM_E_ARG, "jac_precision must be 64 or 32");
+    if (cfg->newton_max_iters < 1 || cfg->newton_max_iters > 0xffff)
+        return set_err(nullptr, CLM_E_ARG, "newton_max_iters out of range");
+    if (cfg->col_ptr[0] != 0) return set_err(nullptr, CLM_E_ARG, "col_ptr[0] must be 0");
+    for (int64_t i = 0; i < cfg->ncol; ++i) {
+        int32_t n = cfg->col_ptr[i + 1] - cfg->col_ptr[i];
+        if (n < 2 || n > 4096)
+            return set_err(nullptr, CLM_E_ARG, "column %lld has %d layers (need 2..4096)",
+                           (long long)i, n);
+    }
+    for (int64_t i = 0; i < cfg->ncol; ++i) {
+        int64_t a = cfg->col_ptr[i] + i, n = cfg->col_ptr[i + 1] - cfg->col_ptr[i];
+        for (int64_t f = 0; f < n; ++f)
+            if (!(cfg->z_face[a + f + 1] > cfg->z_face[a + f]))
+                return set_err(nullptr, CLM_E_ARG, "column %lld: faces not increasing",
+                               (long long)i);
+    }
+
+    clm_ctx* c = new clm_ctx();
+    c->model = cfg->model;
+    c->bc_bottom = cfg->bc_bottom;
+    c->jac_prec = cfg->jac_precision;
+    c->max_iters = cfg->newton_max_iters;
+    c->tol = cfg->newton_tol;
+    c->ncol = cfg->ncol;
+    c->nparam = need;
+    c->col_ptr.assign(cfg->col_ptr, cfg->col_ptr + cfg->ncol + 1);
+    c->ncell = c->col_ptr.back();
+
+    // ---- layout: sort by layer count (descending), packs of 64 ----
+    std::vector<int32_t> order((size_t)c->ncol);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return (c->col_ptr[a + 1] - c->col_ptr[a]) > (c->col_ptr[b + 1] - c->col_ptr[b]);
+    });
+    c->npack = (int32_t)((c->ncol + 63) / 64);
+    c->nslot = (int64_t)c->npack * 64;
+    c->slot_col.assign((size_t)c->nslot, -1);
+    c->slot_nl.assign((size_t)c->nslot, 0);
+    for (int64_t k = 0; k < c->ncol; ++k) {
+        c->slot_col[k] = order[k];
+        c->slot_nl[k] = c->col_ptr[order[k] + 1] - c->col_ptr[order[k]];
+    }
+    c->cell_off.assign((size_t)c->npack + 1, 0);
+    c->face_off.assign((size_t)c->npack + 1, 0);
+    for (int32_t p = 0; p < c->npack; ++p) {
+        int32_t nmax = 0;
+        for (int l = 0; l < 64; ++l) nmax = std::max(nmax, c->slot_nl[(int64_t)p * 64 + l]);
+        c->cell_off[p + 1] = c->cell_off[p] + 64LL * nmax;
+        c->face_off[p + 1] = c->face_off[p] + 64LL * (nmax + 1);
+    }
+    c->npcell = c->cell_off.back();
+    c->npface = c->face_off.back();
+
+    if (hip_stream) {
+        c->stream = (hipStream_t)hip_stream;
+        c->own_stream = false;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return set_err(nullptr, CLM_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        }
+        c->own_stream = true;
+    }
+    int r = 0;
+    auto fail = [&](int code) {
+        g_create_error = c->err;
+        clm_destroy(c);
+        return code;
+    };
+#define ALLOC(p, n)                          \
+    if ((r = dalloc(c, &(p), (n)))) return fail(r);
+    ALLOC(c->d_cell_off, c->npack + 1);
+    ALLOC(c->d_face_off, c->npack + 1);
+    ALLOC(c->d_nl, c->nslot);
+    ALLOC(c->d_slot_col, c->nslot);
+    ALLOC(c->d_par, (int64_t)NPARAM_EH * c->nslot);
+    ALLOC(c->d_zf, c->npface);
+    ALLOC(c->d_xv, c->npcell);
+    ALLOC(c->d_iw, c->nslot);
+    ALLOC(c->d_ftw, c->nslot);
+    ALLOC(c->d_fth, c->nslot);
+    ALLOC(c->d_fbw, c->nslot);
+    ALLOC(c->d_fbh, c->nslot);
+    ALLOC(c->d_K, c->npcell);
+    ALLOC(c->d_psi, c->npcell);
+    ALLOC(c->d_tw, c->nslot);
+    ALLOC(c->d_th, c->nslot);
+    ALLOC(c->d_bw, c->nslot);
+    ALLOC(c->d_bh, c->nslot);
+    ALLOC(c->d_bad, c->nslot);
+    ALLOC(c->d_U0v, c->npcell);
+    ALLOC(c->d_stats, 1);
+    for (int i = 0; i < 5; ++i) {
+        bool cell = i < 3;
+        if (!is_eh(c) && (i == 1 || i == 2 || i == 4)) continue;
+        ALLOC(c->d_wy[i], cell ? c->npcell : c->nslot);
+        ALLOC(c->d_wd[i], cell ? c->npcell : c->nslot);
+    }
+    if (is_eh(c)) {
+        ALLOC(c->d_xi, c->npcell);
+        ALLOC(c->d_xe, c->npcell);
+        ALLOC(c->d_ie, c->nslot);
+        ALLOC(c->d_thl, c->npcell);
+        ALLOC(c->d_T, c->npcell);
+        ALLOC(c->d_kap, c->npcell);
+        ALLOC(c->d_U0e, c->npcell);
+        ALLOC(c->d_Kl, c->npcell);
+        ALLOC(c->d_kl, c->npcell);
+        ALLOC(c->d_tl, c->npcell);
+        ALLOC(c->d_Tp, c->npcell);
+        ALLOC(c->d_Pp, c->npcell);
+        ALLOC(c->d_Re, c->npcell);
+    }
+#undef ALLOC
+    {
+        size_t wbytes = jt_size(c) * (size_t)c->npcell * (is_eh(c) ? 9 : 3);
+        size_t sbytes = jt_size(c) * (size_t)c->npcell * 6;
+        hipError_t e1 = hipMalloc(&c->d_W, wbytes);
+        hipError_t e2 = hipMalloc(&c->d_scr, sbytes);
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            set_err(c, CLM_E_HIP, "hipMalloc of Jacobian/scratch failed");
+            return fail(CLM_E_HIP);
+        }
+        hipMemset(c->d_W, 0, wbytes);
+        hipMemset(c->d_scr, 0, sbytes);
+    }
+    // ---- upload layout, parameters, faces ----
+    {
+        std::vector<double> par((size_t)NPARAM_EH * c->nslot, 0.0);
+        for (int p = 0; p < need; ++p)
+            for (int64_t s = 0; s < c->nslot; ++s) {
+                int32_t col = c->slot_col[s];
+                par[(size_t)p * c->nslot + s] =
+                    col >= 0 ? cfg->params[(int64_t)p * c->ncol + col] : 1.0;
+            }
+        std::vector<double> zf((size_t)c->npface, 0.0);
+        for (int64_t s = 0; s < c->nslot; ++s) {
+            int32_t col = c->slot_col[s];
+            if (col < 0) continue;
+            int64_t base = c->face_off[s >> 6] + (s & 63);
+            const double* src = cfg->z_face + c->col_ptr[col] + col;
+            for (int32_t f = 0; f <= c->slot_nl[s]; ++f) zf[base + 64 * f] = src[f];
+        }
+        hipError_t e = hipSuccess;
+        e = e ? e : hipMemcpy(c->d_cell_off, c->cell_off.data(), 8 * c->cell_off.size(), hipMemcpyHostToDevice);
+        e = e ? e : hipMemcpy(c->d_face_off, c->face_off.data(), 8 * c->face_off.size(), hipMemcpyHostToDevice);
+        e = e ? e : hipMemcpy(c->d_nl, c->slot_nl.data(), 4 * c->slot_nl.size(), hipMemcpyHostToDevice);
+        e = e ? e : hipMemcpy(c->d_slot_col, c->slot_col.data(), 4 * c->slot_col.size(), hipMemcpyHostToDevice);
+        e = e ? e : hipMemcpy(c->d_par, par.data(), 8 * par.size(), hipMemcpyHostToDevice);
+        e = e ? e : hipMemcpy(c->d_zf, zf.data(), 8 * zf.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            set_err(c, CLM_E_HIP, "upload failed: %s", hipGetErrorString(e));
+            return fail(CLM_E_HIP);
+        }
+    }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        set_err(c, CLM_E_HIP, "hipEventCreate failed");
+        return fail(CLM_E_HIP);
+    }
+    *out = c;
+    return CLM_OK;
+}
+
+int clm_destroy(clm_ctx* c) {
+    if (!c) return CLM_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    destroy_graph(c);
+    dfree(c->d_cell_off);
+    dfree(c->d_face_off);
+    dfree(c->d_nl);
+    dfree(c->d_slot_col);
+    dfree(c->d_par);
+    dfree(c->d_zf);
+    dfree(c->d_xv);
+    dfree(c->d_xi);
+    dfree(c->d_xe);
+    dfree(c->d_iw);
+    dfree(c->d_ie);
+    dfree(c->d_ftw);
+    dfree(c->d_fth);
+    dfree(c->d_fbw);
+    dfree(c->d_fbh);
+    dfree(c->d_tab_w);
+    dfree(c->d_tab_h);
+    dfree(c->d_K);
+    dfree(c->d_psi);
+    dfree(c->d_thl);
+    dfree(c->d_T);
+    dfree(c->d_kap);
+    dfree(c->d_tw);
+    dfree(c->d_th);
+    dfree(c->d_bw);
+    dfree(c->d_bh);
+    for (int i = 0; i < 5; ++i) {
+        dfree(c->d_wy[i]);
+        dfree(c->d_wd[i]);
+    }
+    dfree(c->d_W);
+    dfree(c->d_bad);
+    dfree(c->d_U0v);
+    dfree(c->d_U0e);
+    dfree(c->d_Kl);
+    dfree(c->d_kl);
+    dfree(c->d_tl);
+    dfree(c->d_Tp);
+    dfree(c->d_Pp);
+    dfree(c->d_Re);
+    dfree(c->d_scr);
+    dfree(c->d_iters);
+    dfree(c->d_stats);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return CLM_OK;
+}
+
+int clm_set_state(clm_ctx* c, const clm_state* Y) {
+    if (!c) return CLM_E_ARG;
+    double* dst[5] = {c->d_xv, c->d_xi, c->d_xe, c->d_iw, c->d_ie};
+    return upload_state(c, Y, dst);
+}
+
+int clm_get_state(clm_ctx* c, clm_state* Y) {
+    if (!c || !Y) return CLM_E_ARG;
+    double* src[5] = {c->d_xv, c->d_xi, c->d_xe, c->d_iw, c->d_ie};
+    return download_state(c, src, Y);
+}
+
+int clm_set_forcing(clm_ctx* c, const double* tw, const double* th, const double* bw,
+                    const double* bh) {
+    if (!c) return CLM_E_ARG;
+    int r;
+    if ((r = upload_cols(c, tw, c->d_ftw))) return r;
+    if ((r = upload_cols(c, th, c->d_fth))) return r;
+    if ((r = upload_cols(c, bw, c->d_fbw))) return r;
+    return upload_cols(c, bh, c->d_fbh);
+}
+
+int clm_set_forcing_table(clm_ctx* c, int32_t nslots, const double* tw, const double* th) {
+    if (!c) return CLM_E_ARG;
+    if (nslots < 1 || !tw) return set_err(c, CLM_E_ARG, "forcing table needs nslots >= 1");
+    if (nslots != c->tab_slots) {
+        dfree(c->d_tab_w);
+        dfree(c->d_tab_h);
+        int r;
+        if ((r = dalloc(c, &c->d_tab_w, (int64_t)nslots * c->nslot))) return r;
+        if ((r = dalloc(c, &c->d_tab_h, (int64_t)nslots * c->nslot))) return r;
+        c->tab_slots = nslots;
+        destroy_graph(c);
+    }
+    for (int32_t k = 0; k < nslots; ++k) {
+        int r;
+        if ((r = upload_cols(c, tw + (int64_t)k * c->ncol, c->d_tab_w + (int64_t)k * c->nslot)))
+            return r;
+        if ((r = upload_cols(c, th ? th + (int64_t)k * c->ncol : nullptr,
+                             c->d_tab_h + (int64_t)k * c->nslot)))
+            return r;
+    }
+    return CLM_OK;
+}
+
+// ---- fine-grained hooks ------------------------------------------------------
+
+int clm_update_cache(clm_ctx* c, const clm_state* Y, double t) {
+    (void)t;
+    if (!c) return CLM_E_ARG;
+    int r = upload_state(c, Y, c->d_wy);
+    if (r) return r;
+    Layout L = layout(c);
+    if (!is_eh(c)) {
+        hipLaunchKernelGGL(k_rre_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
+                           c->d_par, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw, c->d_tw,
+                           c->d_bw, c->bc_bottom, 1);
+    } else {
+        hipLaunchKernelGGL(k_eh_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
+                           c->d_par, c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_thl, c->d_kap,
+                           c->d_T, c->d_K, c->d_psi, c->d_ftw, c->d_fth, c->d_fbw, c->d_fbh,
+                           c->d_tw, c->d_th, c->d_bw, c->d_bh, c->bc_bottom);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_cache = true;
+    return CLM_OK;
+}
+
+int clm_update_implicit_cache(clm_ctx* c, const clm_state* Y, double t) {
+    (void)t;
+    if (!c) return CLM_E_ARG;
+    if (!c->have_cache) return set_err(c, CLM_E_STATE, "clm_update_cache must run first");
+    int r = upload_state(c, Y, c->d_wy);
+    if (r) return r;
+    Layout L = layout(c);
+    if (!is_eh(c)) {
+        // update_aux (models.jl:207-210); boundary fluxes stay lagged (rre.jl:460-468)
+        hipLaunchKernelGGL(k_rre_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
+                           c->d_par, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw, c->d_tw,
+                           c->d_bw, c->bc_bottom, 0);
+    } else {
+        hipLaunchKernelGGL(k_eh_imp_cache, grid_slots(c), dim3(256), 0, c->stream, L, c->d_par,
+                           c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_T, c->d_psi);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int clm_exp_tendency(clm_ctx* c, const clm_state* Y, double t, clm_state* dY) {
+    if (!c || !dY) return CLM_E_ARG;
+    // make_exp_tendency = update_cache! + compute_exp_tendency! (models.jl:279-287)
+    int r = clm_update_cache(c, Y, t);
+    if (r) return r;
+    Layout L = layout(c);
+    if (!is_eh(c)) {
+        HIPCHK(c, hipMemsetAsync(c->d_wd[0], 0, 8 * c->npcell, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_wd[3], 0, 8 * c->nslot, c->stream));
+    } else {
+        HIPCHK(c, hipMemsetAsync(c->d_bad, 0, 4 * c->nslot, c->stream));
+        hipLaunchKernelGGL(k_eh_exp_tendency, grid_slots(c), dim3(256), 0, c->stream, L,
+                           c->d_par, c->d_zf, c->d_wy[1], c->d_thl, c->d_T, c->d_kap,
+                           c->d_wd[0], c->d_wd[1], c->d_wd[2], c->d_wd[3], c->d_wd[4],
+                           c->d_bad);
+        HIPCHK(c, hipGetLastError());
+    }
+    r = download_state(c, c->d_wd, dY);
+    if (r) return r;
+    if (is_eh(c)) {
+        std::vector<int32_t> bad((size_t)c->nslot);
+        HIPCHK(c, hipMemcpy(bad.data(), c->d_bad, 4 * bad.size(), hipMemcpyDeviceToHost));
+        for (auto b : bad)
+            if (b)
+                return set_err(c, CLM_E_DOMAIN,
+                               "inverse_matric_potential: matric potential is positive");
+    }
+    return CLM_OK;
+}
+
+int clm_imp_tendency(clm_ctx* c, const clm_state* Y, double t, clm_state* dY) {
+    (void)t;
+    (void)Y;
+    if (!c || !dY) return CLM_E_ARG;
+    if (!c->have_cache) return set_err(c, CLM_E_STATE, "clm_update_cache must run first");
+    Layout L = layout(c);
+    if (!is_eh(c)) {
+        hipLaunchKernelGGL(k_rre_imp_tendency, grid_slots(c), dim3(256), 0, c->stream, L,
+                           c->d_zf, c->d_K, c->d_psi, c->d_tw, c->d_bw, c->d_wd[0],
+                           c->d_wd[3]);
+    } else {
+        hipLaunchKernelGGL(k_eh_imp_tendency, grid_slots(c), dim3(256), 0, c->stream, L,
+                           c->d_zf, c->d_K, c->d_psi, c->d_T, c->d_kap, c->d_tw, c->d_th,
+                           c->d_bw, c->d_bh, c->d_wd[0], c->d_wd[1], c->d_wd[2], c->d_wd[3],
+                           c->d_wd[4]);
+    }
+    HIPCHK(c, hipGetLastError());
+    return download_state(c, c->d_wd, dY);
+}
+
+int clm_jacobian(clm_ctx* c, const clm_state* Y, double dtgamma, double t) {
+    (void)t;
+    if (!c) return CLM_E_ARG;
+    if (!c->have_cache) return set_err(c, CLM_E_STATE, "clm_update_cache must run first");
+    int r = upload_state(c, Y, c->d_wy);
+    if (r) return r;
+    Layout L = layout(c);
+    int64_t n = c->npcell;
+    if (!is_eh(c)) {
+        if (c->jac_prec == 32) {
+            float* W = (float*)c->d_W;
+            hipLaunchKernelGGL(k_rre_jacobian<float>, grid_slots(c), dim3(256), 0, c->stream, L,
+                               c->d_par, c->d_zf, c->d_wy[0], c->d_K, dtgamma, W, W + n,
+                               W + 2 * n);
+        } else {
+            double* W = (double*)c->d_W;
+            hipLaunchKernelGGL(k_rre_jacobian<double>, grid_slots(c), dim3(256), 0, c->stream,
+                               L, c->d_par, c->d_zf, c->d_wy[0], c->d_K, dtgamma, W, W + n,
+                               W + 2 * n);
+        }
+    } else {
+        if (c->jac_prec == 32) {
+            float* W = (float*)c->d_W;
+            hipLaunchKernelGGL(k_eh_jacobian<float>, grid_slots(c), dim3(256), 0, c->stream, L,
+                               c->d_par, c->d_zf, c->d_wy[0], c->d_wy[1], c->d_K, c->d_T,
+                               c->d_kap, c->d_thl, dtgamma, W, W + 3 * n, W + 6 * n, n);
+        } else {
+            double* W = (double*)c->d_W;
+            hipLaunchKernelGGL(k_eh_jacobian<double>, grid_slots(c), dim3(256), 0, c->stream,
+                               L, c->d_par, c->d_zf, c->d_wy[0], c->d_wy[1], c->d_K, c->d_T,
+                               c->d_kap, c->d_thl, dtgamma, W, W + 3 * n, W + 6 * n, n);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_jac = true;
+    return CLM_OK;
+}
+
+int clm_solve(clm_ctx* c, clm_state* dx, const clm_state* R) {
+    if (!c || !dx || !R) return CLM_E_ARG;
+    if (!c->have_jac) return set_err(c, CLM_E_STATE, "clm_jacobian must run first");
+    int r = upload_state(c, R, c->d_wy);
+    if (r) return r;
+    Layout L = layout(c);
+    int64_t n = c->npcell;
+    if (!is_eh(c)) {
+        if (c->jac_prec == 32) {
+            float* W = (float*)c->d_W;
+            float* s = (float*)c->d_scr;
+            hipLaunchKernelGGL(k_rre_solve<float>, grid_slots(c), dim3(256), 0, c->stream, L, W,
+                               W + n, W + 2 * n, c->d_wy[0], c->d_wy[3], c->d_wd[0],
+                               c->d_wd[3], s, s + n);
+        } else {
+            double* W = (double*)c->d_W;
+            double* s = (double*)c->d_scr;
+            hipLaunchKernelGGL(k_rre_solve<double>, grid_slots(c), dim3(256), 0, c->stream, L,
+                               W, W + n, W + 2 * n, c->d_wy[0], c->d_wy[3], c->d_wd[0],
+                               c->d_wd[3], s, s + n);
+        }
+    } else {
+        if (c->jac_prec == 32) {
+            float* W = (float*)c->d_W;
+            float* s = (float*)c->d_scr;
+            hipLaunchKernelGGL(k_eh_solve<float>, grid_slots(c), dim3(256), 0, c->stream, L, W,
+                               W + 3 * n, W + 6 * n, n, c->d_wy[0], c->d_wy[1], c->d_wy[2],
+                               c->d_wy[3], c->d_wy[4], c->d_wd[0], c->d_wd[1], c->d_wd[2],
+                               c->d_wd[3], c->d_wd[4], s, s + n, c->d_Re);
+        } else {
+            double* W = (double*)c->d_W;
+            double* s = (double*)c->d_scr;
+            hipLaunchKernelGGL(k_eh_solve<double>, grid_slots(c), dim3(256), 0, c->stream, L,
+                               W, W + 3 * n, W + 6 * n, n, c->d_wy[0], c->d_wy[1], c->d_wy[2],
+                               c->d_wy[3], c->d_wy[4], c->d_wd[0], c->d_wd[1], c->d_wd[2],
+                               c->d_wd[3], c->d_wd[4], s, s + n, c->d_Re);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    return download_state(c, c->d_wd, dx);
+}
+
+int clm_get_aux(clm_ctx* c, clm_aux* p) {
+    if (!c || !p) return CLM_E_ARG;
+    if (!c->have_cache) return set_err(c, CLM_E_STATE, "clm_update_cache must run first");
+    int r;
+    if ((r = download_cells(c, c->d_K, p->K))) return r;
+    if ((r = download_cells(c, c->d_psi, p->psi))) return r;
+    if ((r = download_cols(c, c->d_tw, p->top_water))) return r;
+    if ((r = download_cols(c, c->d_bw, p->bottom_water))) return r;
+    if (is_eh(c)) {
+        if ((r = download_cells(c, c->d_thl, p->theta_l))) return r;
+        if ((r = download_cells(c, c->d_T, p->T))) return r;
+        if ((r = download_cells(c, c->d_kap, p->kappa))) return r;
+        if ((r = download_cols(c, c->d_th, p->top_heat))) return r;
+        if ((r = download_cols(c, c->d_bh, p->bottom_heat))) return r;
+    }
+    return CLM_OK;
+}
+
+int clm_get_jacobian(clm_ctx* c, double* W_ww, double* W_ee, double* W_ew) {
+    if (!c) return CLM_E_ARG;
+    if (!c->have_jac) return set_err(c, CLM_E_STATE, "clm_jacobian must run first");
+    int nplanes = is_eh(c) ? 9 : 3;
+    size_t n = (size_t)c->npcell;
+    std::vector<double> w(n * nplanes);
+    if (c->jac_prec == 32) {
+        std::vector<float> f(n * nplanes);
+        HIPCHK(c, hipMemcpy(f.data(), c->d_W, 4 * f.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); ++i) w[i] = f[i];
+    } else {
+        HIPCHK(c, hipMemcpy(w.data(), c->d_W, 8 * w.size(), hipMemcpyDeviceToHost));
+    }
+    double* outs[3] = {W_ww, W_ee, W_ew};
+    for (int b = 0; b < nplanes / 3; ++b) {
+        double* o = outs[b];
+        if (!o) continue;
+        for (int64_t s = 0; s < c->nslot; ++s) {
+            int32_t col = c->slot_col[s];
+            if (col < 0) continue;
+            int64_t base = c->cell_off[s >> 6] + (s & 63);
+            for (int32_t j = 0; j < c->slot_nl[s]; ++j)
+                for (int k = 0; k < 3; ++k)
+                    o[3 * (int64_t)(c->col_ptr[col] + j) + k] =
+                        w[(size_t)(3 * b + k) * n + base + 64 * j];
+        }
+    }
+    return CLM_OK;
+}
+
+// ---- coarse path ---------------------------------------------------------------
+
+int clm_step(clm_ctx* c, double t, double dt, clm_step_stats* stats) {
+    (void)t;
+    if (!c) return CLM_E_ARG;
+    if (!(dt > 0.0)) return set_err(c, CLM_E_ARG, "dt must be > 0");
+    int r = ensure_iters(c, 1);
+    if (r) return r;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    launch_step(c, c->d_ftw, c->d_fth, dt, c->d_iters);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->have_cache = false;  // the fused step does not maintain the fine-hook cache
+    c->have_jac = false;
+    if (stats) {
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms = ms;
+        c->last_launches = 1;
+        if ((r = collect_stats(c, 1, stats))) return r;
+        stats->ms = ms;
+        if (stats->n_domain_error)
+            return set_err(c, CLM_E_DOMAIN,
+                           "inverse_matric_potential: matric potential is positive");
+    }
+    return CLM_OK;
+}
+
+int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_slot,
+            clm_step_stats* stats, int32_t* iters_out) {
+    (void)t0;
+    if (!c) return CLM_E_ARG;
+    if (nsteps < 1 || steps_per_slot < 1 || !(dt > 0.0))
+        return set_err(c, CLM_E_ARG, "clm_run: bad nsteps/steps_per_slot/dt");
+    if (c->tab_slots < 1) return set_err(c, CLM_E_STATE, "clm_set_forcing_table must run first");
+    int r = ensure_iters(c, nsteps);
+    if (r) return r;
+    if (!c->gexec || c->g_nsteps != nsteps || c->g_sps != steps_per_slot ||
+        c->g_tab != c->tab_slots || c->g_dt != dt) {
+        destroy_graph(c);
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        for (int32_t s = 0; s < nsteps; ++s) {
+            int64_t slot = (s / steps_per_slot) % c->tab_slots;
+            launch_step(c, c->d_tab_w + slot * c->nslot, c->d_tab_h + slot * c->nslot, dt,
+                        c->d_iters + (int64_t)s * c->nslot);
+        }
+        hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
+        if (e != hipSuccess) {
+            c->graph = nullptr;
+            return set_err(c, CLM_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        }
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->g_nsteps = nsteps;
+        c->g_sps = steps_per_slot;
+        c->g_tab = c->tab_slots;
+        c->g_dt = dt;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->have_cache = false;
+    c->have_jac = false;
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = nsteps;
+    if (stats) {
+        if ((r = collect_stats(c, nsteps, stats))) return r;
+        stats->ms = ms;
+    }
+    if (iters_out) {
+        std::vector<int32_t> h((size_t)nsteps * c->nslot);
+        HIPCHK(c, hipMemcpy(h.data(), c->d_iters, 4 * h.size(), hipMemcpyDeviceToHost));
+        for (int32_t s = 0; s < nsteps; ++s)
+            for (int64_t k = 0; k < c->nslot; ++k)
+                if (c->slot_col[k] >= 0)
+                    iters_out[(int64_t)s * c->ncol + c->slot_col[k]] = h[(size_t)s * c->nslot + k];
+    }
+    if (stats && stats->n_domain_error)
+        return set_err(c, CLM_E_DOMAIN, "inverse_matric_potential: matric potential is positive");
+    return CLM_OK;
+}
+
+int clm_totals(clm_ctx* c, double* water, double* energy) {
+    if (!c || !water) return CLM_E_ARG;
+    std::vector<double> v((size_t)c->ncell), t, e;
+    clm_state Y = {v.data(), nullptr, nullptr, nullptr, nullptr};
+    std::vector<double> iw((size_t)c->ncol), ie((size_t)c->ncol);
+    Y.intF_water = iw.data();
+    if (is_eh(c)) {
+        t.resize((size_t)c->ncell);
+        e.resize((size_t)c->ncell);
+        Y.theta_i = t.data();
+        Y.rho_e_int = e.data();
+        Y.intF_energy = ie.data();
+    }
+    int r = clm_get_state(c, &Y);
+    if (r) return r;
+    // zf in CSR order from the packed copy
+    std::vector<double> zf((size_t)c->npface);
+    HIPCHK(c, hipMemcpy(zf.data(), c->d_zf, 8 * zf.size(), hipMemcpyDeviceToHost));
+    for (int64_t s = 0; s < c->nslot; ++s) {
+        int32_t col = c->slot_col[s];
+        if (col < 0) continue;
+        int64_t fb = c->face_off[s >> 6] + (s & 63);
+        double w = 0.0, en = 0.0;
+        for (int32_t j = 0; j < c->slot_nl[s]; ++j) {
+            double dz = zf[fb + 64 * (j + 1)] - zf[fb + 64 * j];
+            int64_t i = c->col_ptr[col] + j;
+            double vol = v[i];
+            if (is_eh(c)) {
+                vol += t[i] * RHO_I / RHO_L;
+                en += e[i] * dz;
+            }
+            w += vol * dz;
+        }
+        water[col] = w;
+        if (energy) energy[col] = en;
+    }
+    return CLM_OK;
+}
+
+int clm_last_kernel_ms(const clm_ctx* c, double* ms_per_launch, int32_t* launches) {
+    if (!c || !ms_per_launch) return CLM_E_ARG;
+    *ms_per_launch = c->last_launches ? c->last_ms / c->last_launches : 0.0;
+    if (launches) *launches = c->last_launches;
+    return CLM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+/*
+ * clm_soil.h -- C ABI of libclmsoil.so: the MI355X (gfx950) implicit
+ * soil-column step of ClimaLand.jl's RichardsModel and EnergyHydrology.
+ *
+ * The entry points replace, one for one, the hooks ClimaTimeSteppers calls
+ * on a soil model (ClimaLand.jl v1.11.2):
+ *
+ *   clm_update_cache          <- make_update_cache(model)(p, Y, t)
+ *                                src/shared_utilities/models.jl:187-195
+ *   clm_exp_tendency          <- T_exp! = make_exp_tendency(model)
+ *                                models.jl:279-287; rre.jl:214-243;
+ *                                energy_hydrology.jl:315-350, 838-906
+ *   clm_imp_tendency          <- T_imp! = make_compute_imp_tendency(model)
+ *                                rre.jl:161-203; energy_hydrology.jl:363-425
+ *   clm_jacobian              <- Wfact = make_compute_jacobian(model)
+ *                                rre.jl:391-458; energy_hydrology.jl:466-576
+ *   clm_solve                 <- ldiv!(Δx, jac_prototype, R) on the
+ *                                FieldMatrixWithSolver built by
+ *                                initialize_jacobian(Y),
+ *                                src/shared_utilities/implicit_timestepping.jl:63-172
+ *   clm_update_implicit_cache <- cache_imp! = make_update_implicit_cache(model)
+ *                                models.jl:238-246; energy_hydrology.jl:427-455
+ *   clm_step / clm_run        <- step!(sim) / solve!(sim) of a LandSimulation
+ *                                with IMEXAlgorithm(ARS111, NewtonsMethod),
+ *                                src/simulations/Simulations.jl:127-135,312-332
+ *
+ * Conventions
+ *   - Host buffers are CSR-packed per column, level 0 = bottom cell:
+ *     cells of column c are [col_ptr[c], col_ptr[c+1]); its N+1 faces are
+ *     z_face[col_ptr[c] + c ... col_ptr[c+1] + c].
+ *   - Per-column parameters are SoA: params[p * ncol + c], rows CLM_P_*.
+ *   - All values are FP64 at the boundary, whatever the internal precision.
+ *   - The library owns device memory; host pointers are borrowed for the
+ *     duration of the call and copied.  Nothing allocates after
+ *     clm_create except clm_set_forcing_table (which may grow its table).
+ *   - Every function returns CLM_OK (0) or a negative CLM_E_* code and
+ *     never aborts; clm_last_error(ctx) gives the message.
+ *   - A context is used from one host thread at a time; its work is
+ *     ordered on its own HIP stream.  Calls that return data to the host
+ *     synchronise that stream.
+ */
+#ifndef CLM_SOIL_H
+#define CLM_SOIL_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLM_ABI_VERSION 1
+
+enum {
+    CLM_OK = 0,
+    CLM_E_ARG = -1,      /* invalid argument / shape */
+    CLM_E_HIP = -2,      /* HIP runtime error (device, memory, launch) */
+    CLM_E_DOMAIN = -3,   /* reference would raise: e.g. inverse_matric_potential(ψ>0) */
+    CLM_E_STATE = -4     /* call order (e.g. clm_solve before clm_jacobian) */
+};
+
+enum { CLM_MODEL_RICHARDS = 0, CLM_MODEL_ENERGY_HYDROLOGY = 1 };
+
+/* Top boundary: prescribed fluxes (WaterFluxBC; EH: WaterHeatBC(WaterFluxBC,
+ * HeatFluxBC)), boundary_conditions.jl:168-180, 465-477, 709-724.
+ * Bottom boundary: */
+enum {
+    CLM_BC_FLUX = 0,          /* prescribed fluxes */
+    CLM_BC_FREE_DRAINAGE = 1  /* FreeDrainage (RRE, :340-353) or
+                                 EnergyWaterFreeDrainage (EH, :590-608) */
+};
+
+/* Per-column parameter rows.  RRE uses the first CLM_NPARAM_RRE rows
+ * (RichardsParameters, rre.jl:7-48); EH all CLM_NPARAM_EH rows
+ * (EnergyHydrologyParameters, energy_hydrology.jl:21-78, with κ_dry,
+ * κ_sat_frozen, κ_sat_unfrozen and ρc_ds as already-derived values). */
+enum {
+    CLM_P_NU = 0,
+    CLM_P_THETA_R,
+    CLM_P_VG_ALPHA,
+    CLM_P_VG_N,
+    CLM_P_KSAT,
+    CLM_P_SS,
+    CLM_P_NU_SS_OM,
+    CLM_P_NU_SS_QUARTZ,
+    CLM_P_NU_SS_GRAVEL,
+    CLM_P_KAPPA_DRY,
+    CLM_P_KAPPA_SAT_FROZEN,
+    CLM_P_KAPPA_SAT_UNFROZEN,
+    CLM_P_RHOC_DS,
+    CLM_NPARAM_EH
+};
+#define CLM_NPARAM_RRE 6
+
+typedef struct clm_ctx clm_ctx;
+
+typedef struct {
+    int32_t model;            /* CLM_MODEL_* */
+    int32_t bc_bottom;        /* CLM_BC_* */
+    int64_t ncol;
+    const int32_t* col_ptr;   /* ncol+1, col_ptr[0] = 0 */
+    const double* z_face;     /* col_ptr[ncol] + ncol face heights, increasing */
+    const double* params;     /* nparam * ncol */
+    int32_t nparam;
+    int32_t jac_precision;    /* 64: FP64 Jacobian+solve; 32: FP32 Jacobian+solve,
+                                 residual/state/closures stay FP64 */
+    int32_t newton_max_iters; /* NewtonsMethod(max_iters) */
+    double newton_tol;        /* <= 0: exactly max_iters iterations (no checker);
+                                 > 0: per-column stop when max|Δx| < tol over
+                                 the cell unknowns (ρe_int scaled by 1/ρc_s) */
+} clm_config;
+
+/* Prognostic state Y.soil.  RRE: vartheta_l, intF_water.
+ * EH: vartheta_l, theta_i, rho_e_int, intF_water, intF_energy
+ * (prognostic_vars, rre.jl:282, energy_hydrology.jl:624-625).
+ * Unused members may be NULL. */
+typedef struct {
+    double* vartheta_l;
+    double* theta_i;
+    double* rho_e_int;
+    double* intF_water;
+    double* intF_energy;
+} clm_state;
+
+/* Cache p.soil members that the hooks read and write (auxiliary_vars,
+ * rre.jl:292-305, energy_hydrology.jl:644-660).  Unused members may be NULL. */
+typedef struct {
+    double* K;
+    double* psi;
+    double* theta_l;
+    double* T;
+    double* kappa;
+    double* top_water;   /* p.soil.top_bc(.water) */
+    double* top_heat;    /* p.soil.top_bc.heat */
+    double* bottom_water;
+    double* bottom_heat;
+} clm_aux;
+
+typedef struct {
+    int32_t max_iters_used;   /* max Newton iterations over columns */
+    int32_t pad_;
+    double mean_iters;        /* mean Newton iterations per column-step */
+    int64_t n_nonconverged;   /* column-steps that hit max_iters with tol > 0 */
+    int64_t n_nan;            /* columns with a non-finite state after the step */
+    int64_t n_domain_error;   /* phase change reached inverse_matric_potential(ψ>0) */
+    double ms;                /* device time of the step(s), HIP events */
+} clm_step_stats;
+
+/* ---- lifetime ---------------------------------------------------------- */
+int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream /* NULL: own */);
+int clm_destroy(clm_ctx* ctx);
+const char* clm_last_error(const clm_ctx* ctx); /* ctx may be NULL: last create error */
+int clm_abi_version(void);
+
+/* ---- state / forcing (host <-> device) -------------------------------- */
+int clm_set_state(clm_ctx* ctx, const clm_state* Y);
+int clm_get_state(clm_ctx* ctx, clm_state* Y);
+/* Prescribed boundary fluxes (the values bc(p, t) returns), per column.
+ * bottom_* are read only for CLM_BC_FLUX.  NULL arrays mean zero. */
+int clm_set_forcing(clm_ctx* ctx, const double* top_water, const double* top_heat,
+                    const double* bottom_water, const double* bottom_heat);
+/* Device-resident forcing schedule for clm_run: nslots rows of ncol values
+ * ([slot][col]); step s of a run uses slot (s / steps_per_slot) % nslots. */
+int clm_set_forcing_table(clm_ctx* ctx, int32_t nslots, const double* top_water,
+                          const double* top_heat);
+
+/* ---- fine-grained hooks (1:1 with the ClimaLand closures) ----------------
+ * Y / dY are host CSR buffers.  The cache p lives on the device inside ctx;
+ * clm_get_aux copies it out. */
+int clm_update_cache(clm_ctx* ctx, const clm_state* Y, double t);
+int clm_exp_tendency(clm_ctx* ctx, const clm_state* Y, double t, clm_state* dY);
+int clm_imp_tendency(clm_ctx* ctx, const clm_state* Y, double t, clm_state* dY);
+int clm_jacobian(clm_ctx* ctx, const clm_state* Y, double dtgamma, double t);
+int clm_solve(clm_ctx* ctx, clm_state* dx, const clm_state* R);
+int clm_update_implicit_cache(clm_ctx* ctx, const clm_state* Y, double t);
+int clm_get_aux(clm_ctx* ctx, clm_aux* p);
+/* Jacobian rows as assembled by clm_jacobian, CSR cells x 3 (lower, diag,
+ * upper) per block: ϑϑ; EH also ρeρe and ρeϑ (NULL to skip).  FP64 even when
+ * jac_precision = 32 (values widened). */
+int clm_get_jacobian(clm_ctx* ctx, double* W_ww, double* W_ee, double* W_ew);
+
+/* ---- coarse path: whole ARS111 steps on the device ---------------------- */
+/* One step of size dt from the state set by clm_set_state, forcing from
+ * clm_set_forcing.  stats may be NULL (then nothing synchronises). */
+int clm_step(clm_ctx* ctx, double t, double dt, clm_step_stats* stats);
+/* nsteps steps with the device forcing table; the whole run is one graph
+ * replay after the first call with the same (nsteps, steps_per_slot).
+ * iters_out: NULL or host [nsteps][ncol] Newton iteration counts. */
+int clm_run(clm_ctx* ctx, double t0, double dt, int32_t nsteps, int32_t steps_per_slot,
+            clm_step_stats* stats, int32_t* iters_out);
+
+/* Column totals Σ ϑ_l Δz (+ ρ_i/ρ_l θ_i Δz) and Σ ρe_int Δz of the current
+ * device state (total_liq_water_vol_per_area!, rre.jl:502-511,
+ * energy_hydrology.jl:1282-1320).  energy may be NULL. */
+int clm_totals(clm_ctx* ctx, double* water, double* energy);
+
+/* Device time of the dominant kernel in the last clm_step/clm_run, per
+ * launch (ms), measured with HIP events on the ctx stream. */
+int clm_last_kernel_ms(const clm_ctx* ctx, double* ms_per_launch, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

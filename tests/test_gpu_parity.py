@@ -1,0 +1,295 @@
+"""GPU parity: libclmsoil.so (through its C ABI) against the CPU FP64 oracle.
+
+Bars (stated here, checked below):
+  * FP64 path: per-cell relative error <= 1e-10 for states (and <= 1e-11 for
+    single closure evaluations / Jacobian entries) on columns that stayed
+    finite; Newton iteration counts identical on >= 99% of column-steps.
+  * FP32-Jacobian path (inexact Newton, FP64 residual): |Δϑ_l| <= 1e-6 and
+    |Δρe_int|/ρc_s <= 1e-6 K against the FP64 oracle at the same tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import column_ok, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _models(batch, jac=64, max_iters=None, tol=None):
+    from clmsoil.model import model_from_batch
+    from oracle.oracle import OracleModel
+    return model_from_batch(batch, jac_precision=jac, max_iters=max_iters, tol=tol), \
+        OracleModel.from_batch(batch)
+
+
+def _cell_mask(batch, colmask):
+    return np.repeat(colmask, batch.nlayers)
+
+
+# ---------------------------------------------------------------- hooks: RRE
+
+def test_rre_hooks_parity(gpu_lib):
+    from clmsoil import inputs
+    from clmsoil.model import (initialize_jacobian, make_compute_imp_tendency,
+                               make_compute_jacobian, make_update_cache)
+    b = inputs.make_batch("rre_131072_uneven", ncol=300)
+    m, o = _models(b)
+    Y = b.copy_state()
+    print("rre hooks: ncol", b.ncol, "ncell", b.ncell, flush=True)
+    make_update_cache(m)(None, Y, 0.0)
+    a = m.aux()
+    oa = o.new_aux()
+    o.update_cache(Y, oa, b.top_water[0].copy())
+    assert rel_err(a["K"], oa["K"], 1e-300).max() <= 1e-11
+    assert rel_err(a["psi"], oa["psi"], 1e-300).max() <= 1e-11
+    assert np.array_equal(a["top_water"], oa["top_water"])
+    assert rel_err(a["bottom_water"], oa["bot_water"], 1e-300).max() <= 1e-11
+    dY = m.new_state()
+    make_compute_imp_tendency(m)(dY, Y, None, 0.0)
+    od = o.imp_tendency(Y, oa)
+    # the tendency is a difference of fluxes: compare against the column's scale
+    scale = np.repeat(np.maximum.reduceat(np.abs(od["vartheta_l"]), b.col_ptr[:-1]), b.nlayers)
+    assert (np.abs(dY["vartheta_l"] - od["vartheta_l"]) <= 1e-10 * scale + 1e-300).all()
+    assert rel_err(dY["intF_water"], od["intF_water"], 1e-300).max() <= 1e-12
+    W = initialize_jacobian(m)
+    make_compute_jacobian(m)(W, Y, None, 1800.0, 0.0)
+    ww, _, _ = m.get_jacobian()
+    oww, _, _ = o.jacobian(Y, oa, 1800.0)
+    assert rel_err(ww, oww, 1e-300).max() <= 1e-11
+    R = m.new_state()
+    rng = np.random.default_rng(7)
+    R["vartheta_l"][:] = rng.standard_normal(b.ncell)
+    R["intF_water"][:] = rng.standard_normal(b.ncol)
+    dx = m.new_state()
+    W.ldiv(dx, R)
+    odx = o.solve((oww, None, None), R)
+    assert rel_err(dx["vartheta_l"], odx["vartheta_l"], 1e-6).max() <= 1e-10
+    assert np.array_equal(dx["intF_water"], -R["intF_water"])
+
+
+# ----------------------------------------------------------------- hooks: EH
+
+def _eh_state_with_ice(b, o, nsteps=6):
+    Y = b.copy_state()
+    for s in range(nsteps):
+        sl = b.forcing_slot(s)
+        o.step(Y, b.workload.dt, 50, 1e-8, b.top_water[sl].copy(), b.top_heat[sl].copy())
+    return Y
+
+
+@pytest.mark.parametrize("jac", [64, 32])
+def test_eh_hooks_parity(gpu_lib, jac):
+    from clmsoil import inputs
+    from clmsoil.model import (initialize_jacobian, make_compute_imp_tendency,
+                               make_compute_jacobian, make_exp_tendency,
+                               make_update_implicit_cache)
+    b = inputs.make_batch("eh_65536x20", ncol=200)
+    m, o = _models(b, jac=jac)
+    Y = _eh_state_with_ice(b, o)
+    ok = column_ok(b, Y)
+    assert ok.mean() > 0.9
+    sub = b.column_subset(np.nonzero(ok)[0])
+    Ys = {k: (v[_cell_mask(b, ok)] if v.size == b.ncell else v[ok]).copy() for k, v in Y.items()}
+    m, o = _models(sub, jac=jac)
+    assert (Ys["theta_i"] > 0).any() and (Ys["theta_i"] == 0).any()
+    print("eh hooks: ncol", sub.ncol, "jac", jac, flush=True)
+    dY = m.new_state()
+    make_exp_tendency(m)(dY, Ys, None, 0.0)
+    a = m.aux()
+    oa = o.new_aux()
+    o.update_cache(Ys, oa, sub.top_water[0].copy(), sub.top_heat[0].copy())
+    for k in ("K", "psi", "T", "kappa", "theta_l"):
+        assert rel_err(a[k], oa[k], 1e-300).max() <= 1e-11, k
+    assert rel_err(a["bottom_heat"], oa["bot_heat"], 1e-300).max() <= 1e-11
+    od = o.exp_tendency(Ys, oa)
+    for k in ("vartheta_l", "theta_i"):
+        sc = np.abs(od[k]).max()
+        assert np.abs(dY[k] - od[k]).max() <= 1e-10 * sc, k
+    imp = m.new_state()
+    make_compute_imp_tendency(m)(imp, Ys, None, 0.0)
+    oi = o.imp_tendency(Ys, oa)
+    for k in ("vartheta_l", "rho_e_int"):
+        scale = np.repeat(np.maximum.reduceat(np.abs(oi[k]), sub.col_ptr[:-1]), sub.nlayers)
+        assert (np.abs(imp[k] - oi[k]) <= 1e-10 * scale + 1e-300).all(), k
+    W = initialize_jacobian(m)
+    make_compute_jacobian(m)(W, Ys, None, 900.0, 0.0)
+    ww, ee, ew = m.get_jacobian()
+    oww, oee, oew = o.jacobian(Ys, oa, 900.0)
+    tolJ = 1e-11 if jac == 64 else 1e-6
+    for g, r in ((ww, oww), (ee, oee), (ew, oew)):
+        assert rel_err(g, r, 1e-300).max() <= tolJ
+    R = m.new_state()
+    rng = np.random.default_rng(3)
+    for k in R:
+        R[k][:] = rng.standard_normal(R[k].size)
+    R["rho_e_int"] *= 1e4
+    dx = m.new_state()
+    W.ldiv(dx, R)
+    odx = o.solve((oww, oee, oew), R)
+    tolS = 1e-9 if jac == 64 else 1e-4
+    for k in ("vartheta_l", "rho_e_int"):
+        sc = np.abs(odx[k]).max()
+        assert np.abs(dx[k] - odx[k]).max() <= tolS * sc, k
+    assert np.array_equal(dx["theta_i"], -R["theta_i"])
+    # cache_imp! refreshes T and ψ only
+    x = {k: v.copy() for k, v in Ys.items()}
+    x["vartheta_l"] += 1e-3
+    make_update_implicit_cache(m)(None, x, 0.0)
+    o.update_implicit_cache(x, oa)
+    a2 = m.aux()
+    for k in ("T", "psi"):
+        assert rel_err(a2[k], oa[k], 1e-300).max() <= 1e-11, k
+    assert np.array_equal(a2["K"], a["K"])
+
+
+# ------------------------------------------------------------ fused steps
+
+def test_rre_step_fixed_iters_per_step(gpu_lib):
+    """rre_4096x50 semantics (3 fixed iterations), 256 columns, every step."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation
+    b = inputs.make_batch("rre_4096x50", ncol=256)
+    m, o = _models(b)
+    sim = LandSimulation(m, b.copy_state(), b.workload.dt, max_iters=3)
+    Yo = b.copy_state()
+    worst = 0.0
+    for s in range(b.workload.nsteps):
+        st = sim.step()
+        assert st.max_iters_used == 3
+        o.step(Yo, b.workload.dt, 3, 0.0, b.top_water[0].copy())
+        Yg = sim.state()
+        ok = column_ok(b, Yo) & column_ok(b, Yg)
+        cm = _cell_mask(b, ok)
+        e = rel_err(Yg["vartheta_l"][cm], Yo["vartheta_l"][cm]).max()
+        worst = max(worst, e)
+        assert e <= 1e-10, (s, e)
+        assert rel_err(Yg["intF_water"][ok], Yo["intF_water"][ok], 1e-12).max() <= 1e-10
+    print("rre fixed-iteration per-step worst rel err", worst, "ok cols", ok.sum(), flush=True)
+    assert ok.mean() > 0.9
+
+
+def _run_both(name, ncol, jac=64, nsteps=None):
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation
+    b = inputs.make_batch(name, ncol=ncol, nsteps=nsteps)
+    w = b.workload
+    m, o = _models(b, jac=jac)
+    sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                         tol=w.newton_tol)
+    st, it_g = sim.solve(b.top_water, b.top_heat, w.nsteps, w.steps_per_slot, want_iters=True)
+    Yo = b.copy_state()
+    it_o, _ = o.run(Yo, b.top_water, b.top_heat, w.steps_per_slot, w.dt, w.nsteps,
+                    w.newton_max_iters, w.newton_tol)
+    return b, sim.state(), Yo, it_g, it_o, st
+
+
+def test_rre_run_tol_parity(gpu_lib):
+    """rre_131072_uneven semantics (Newton to 1e-8), 512 columns, 48 steps, graph run."""
+    b, Yg, Yo, it_g, it_o, st = _run_both("rre_131072_uneven", 512)
+    kg, ko = it_g & 0xFFFF, it_o
+    same = (kg == ko).all(axis=0)
+    conv = ~((it_g & 0x10000) != 0).any(axis=0) & (ko < 50).all(axis=0)
+    print(f"rre tol: iteration counts equal on {(kg == ko).mean():.4f} of column-steps; "
+          f"mean iters {kg.mean():.2f}; converged columns {conv.mean():.3f}", flush=True)
+    assert (kg == ko).mean() >= 0.99
+    ok = same & conv & column_ok(b, Yo) & column_ok(b, Yg)
+    cm = _cell_mask(b, ok)
+    e = rel_err(Yg["vartheta_l"][cm], Yo["vartheta_l"][cm]).max()
+    print("rre tol: max rel err on", ok.sum(), "columns:", e, flush=True)
+    assert e <= 1e-10
+    assert st.n_nonconverged == int(((it_g & 0x10000) != 0).sum())
+
+
+def test_eh_run_fp64_parity(gpu_lib):
+    """eh_65536x20 semantics with an FP64 Jacobian, 256 columns, 96 steps."""
+    b, Yg, Yo, it_g, it_o, st = _run_both("eh_65536x20", 256, jac=64)
+    kg = it_g & 0xFFFF
+    same = (kg == it_o).all(axis=0)
+    conv = ~((it_g & 0x30000) != 0).any(axis=0) & (it_o < 50).all(axis=0)
+    print(f"eh fp64: equal iteration counts {(kg == it_o).mean():.4f}; mean iters "
+          f"{kg.mean():.2f}; converged+finite columns {conv.mean():.3f}", flush=True)
+    assert (kg == it_o).mean() >= 0.99
+    ok = same & conv & column_ok(b, Yo) & column_ok(b, Yg)
+    cm = _cell_mask(b, ok)
+    for k in ("vartheta_l", "theta_i", "rho_e_int"):
+        floor = 1e-12 if k != "rho_e_int" else 1.0
+        e = rel_err(Yg[k][cm], Yo[k][cm], floor).max()
+        print(f"  {k}: max rel err {e:.3g} on {ok.sum()} columns", flush=True)
+        assert e <= 1e-10, k
+
+
+def test_eh_run_mixed_error(gpu_lib):
+    """FP32 Jacobian + solve, FP64 residual: error vs the FP64 oracle at 1e-8."""
+    b, Yg, Yo, it_g, it_o, st = _run_both("eh_65536x20", 256, jac=32)
+    conv = ~((it_g & 0x30000) != 0).any(axis=0) & (it_o < 50).all(axis=0)
+    ok = conv & column_ok(b, Yo) & column_ok(b, Yg)
+    cm = _cell_mask(b, ok)
+    dv = np.abs(Yg["vartheta_l"][cm] - Yo["vartheta_l"][cm]).max()
+    rcs = np.repeat(b.params[12], b.nlayers)[cm] + 4.181e6 * Yo["vartheta_l"][cm]
+    de = (np.abs(Yg["rho_e_int"][cm] - Yo["rho_e_int"][cm]) / rcs).max()
+    print(f"eh mixed: |dϑ| {dv:.3g}, |dρe|/ρc {de:.3g} K on {ok.sum()} columns; mean iters "
+          f"gpu {(it_g & 0xFFFF).mean():.2f} oracle {it_o.mean():.2f}", flush=True)
+    assert dv <= 1e-6 and de <= 1e-6
+
+
+def test_hook_path_matches_fused_step(gpu_lib):
+    """A step driven through the fine hooks (CTS order) equals the fused step."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation
+    for name in ("rre_4096x50", "eh_65536x20"):
+        b = inputs.make_batch(name, ncol=130)
+        m, _ = _models(b, max_iters=3, tol=0.0)
+        s1 = LandSimulation(m, b.copy_state(), b.workload.dt, max_iters=3)
+        s1.step()
+        Yf = {k: v.copy() for k, v in s1.state().items()}
+        m2, _ = _models(b, max_iters=3, tol=0.0)
+        s2 = LandSimulation(m2, b.copy_state(), b.workload.dt, max_iters=3)
+        s2.step_hooks()
+        Yh = s2.state()
+        ok = column_ok(b, Yf)
+        cm = _cell_mask(b, ok)
+        for k in Yf:
+            if Yf[k].size == b.ncell:
+                e = rel_err(Yh[k][cm], Yf[k][cm], 1.0 if k == "rho_e_int" else 1e-12).max()
+                assert e <= 1e-11, (name, k, e)
+
+
+def test_graph_run_deterministic_and_equals_steps(gpu_lib):
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation
+    b = inputs.make_batch("rre_131072_uneven", ncol=200, nsteps=12)
+    w = b.workload
+    outs = []
+    for _ in range(2):
+        m, _ = _models(b)
+        sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                             tol=w.newton_tol)
+        sim.solve(b.top_water, b.top_heat, 12, w.steps_per_slot)
+        outs.append(sim.state()["vartheta_l"].copy())
+    assert np.array_equal(outs[0], outs[1])
+    m, _ = _models(b)
+    sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                         tol=w.newton_tol)
+    for s in range(12):
+        m.boundary_conditions["top"].bc = b.top_water[b.forcing_slot(s)]
+        m.set_forcing()
+        sim.step()
+    assert np.array_equal(sim.state()["vartheta_l"], outs[0])
+
+
+def test_water_balance_on_gpu(gpu_lib):
+    """Σ Δz Δϑ = ∫F over a run, per column (water conservation)."""
+    from clmsoil import inputs
+    b, Yg, Yo, it_g, it_o, st = _run_both("rre_131072_uneven", 256)
+    dz = np.diff(b.z_face)
+    keep = np.ones(dz.size, bool)
+    keep[b.col_ptr[1:] + np.arange(b.ncol) ] = False  # drop the face gaps between columns
+    dz = dz[keep]
+    w0 = np.add.reduceat(b.vartheta_l * dz, b.col_ptr[:-1])
+    w1 = np.add.reduceat(Yg["vartheta_l"] * dz, b.col_ptr[:-1])
+    conv = ~((it_g & 0x30000) != 0).any(axis=0)
+    err = np.abs((w1 - w0) - Yg["intF_water"])[conv]
+    print("water balance: max err", err.max(), "over", conv.sum(), "columns", flush=True)
+    assert err.max() <= 1e-9 * max(1.0, np.abs(Yg["intF_water"]).max())
