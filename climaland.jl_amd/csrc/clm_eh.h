@@ -505,7 +505,8 @@ __global__ __launch_bounds__(256) void k_eh_step(
             }
         }
         conv = (tol > 0.0) && (err < tol);
-        if (conv) break;
+        // a non-finite iterate stays non-finite: stop (the state is NaN either way)
+        if (conv || !isfinite(err)) break;
     }
     if (k > max_iters) k = max_iters;
     int32_t flag = k;

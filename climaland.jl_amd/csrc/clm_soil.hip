@@ -106,15 +106,18 @@ int set_err(clm_ctx* c, int code, const char* fmt, ...) {
                            hipGetErrorString(e_), __FILE__, __LINE__);                   \
     } while (0)
 
+// Allocate and zero on the context stream (stream-ordered with all later
+// work of the context; the null stream does not order against it).
 template <typename T>
 int dalloc(clm_ctx* c, T** p, int64_t n) {
     size_t bytes = sizeof(T) * (size_t)std::max<int64_t>(n, 1);
     hipError_t e = hipMalloc((void**)p, bytes);
     if (e != hipSuccess)
         return set_err(c, CLM_E_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    e = hipMemset(*p, 0, bytes);
+    e = hipMemsetAsync(*p, 0, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess)
-        return set_err(c, CLM_E_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+        return set_err(c, CLM_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
     return CLM_OK;
 }
 
@@ -509,8 +512,8 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
             set_err(c, CLM_E_HIP, "hipMalloc of Jacobian/scratch failed");
             return fail(CLM_E_HIP);
         }
-        hipMemset(c->d_W, 0, wbytes);
-        hipMemset(c->d_scr, 0, sbytes);
+        hipMemsetAsync(c->d_W, 0, wbytes, c->stream);
+        hipMemsetAsync(c->d_scr, 0, sbytes, c->stream);
     }
     // ---- upload layout, parameters, faces ----
     {
@@ -530,12 +533,16 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
             for (int32_t f = 0; f <= c->slot_nl[s]; ++f) zf[base + 64 * f] = src[f];
         }
         hipError_t e = hipSuccess;
-        e = e ? e : hipMemcpy(c->d_cell_off, c->cell_off.data(), 8 * c->cell_off.size(), hipMemcpyHostToDevice);
-        e = e ? e : hipMemcpy(c->d_face_off, c->face_off.data(), 8 * c->face_off.size(), hipMemcpyHostToDevice);
-        e = e ? e : hipMemcpy(c->d_nl, c->slot_nl.data(), 4 * c->slot_nl.size(), hipMemcpyHostToDevice);
-        e = e ? e : hipMemcpy(c->d_slot_col, c->slot_col.data(), 4 * c->slot_col.size(), hipMemcpyHostToDevice);
-        e = e ? e : hipMemcpy(c->d_par, par.data(), 8 * par.size(), hipMemcpyHostToDevice);
-        e = e ? e : hipMemcpy(c->d_zf, zf.data(), 8 * zf.size(), hipMemcpyHostToDevice);
+        auto up = [&](void* d, const void* h, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream);
+        };
+        up(c->d_cell_off, c->cell_off.data(), 8 * c->cell_off.size());
+        up(c->d_face_off, c->face_off.data(), 8 * c->face_off.size());
+        up(c->d_nl, c->slot_nl.data(), 4 * c->slot_nl.size());
+        up(c->d_slot_col, c->slot_col.data(), 4 * c->slot_col.size());
+        up(c->d_par, par.data(), 8 * par.size());
+        up(c->d_zf, zf.data(), 8 * zf.size());
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) {
             set_err(c, CLM_E_HIP, "upload failed: %s", hipGetErrorString(e));
             return fail(CLM_E_HIP);
@@ -714,7 +721,9 @@ int clm_exp_tendency(clm_ctx* c, const clm_state* Y, double t, clm_state* dY) {
     if (r) return r;
     if (is_eh(c)) {
         std::vector<int32_t> bad((size_t)c->nslot);
-        HIPCHK(c, hipMemcpy(bad.data(), c->d_bad, 4 * bad.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpyAsync(bad.data(), c->d_bad, 4 * bad.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         for (auto b : bad)
             if (b)
                 return set_err(c, CLM_E_DOMAIN,
@@ -850,10 +859,14 @@ int clm_get_jacobian(clm_ctx* c, double* W_ww, double* W_ee, double* W_ew) {
     std::vector<double> w(n * nplanes);
     if (c->jac_prec == 32) {
         std::vector<float> f(n * nplanes);
-        HIPCHK(c, hipMemcpy(f.data(), c->d_W, 4 * f.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpyAsync(f.data(), c->d_W, 4 * f.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < f.size(); ++i) w[i] = f[i];
     } else {
-        HIPCHK(c, hipMemcpy(w.data(), c->d_W, 8 * w.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpyAsync(w.data(), c->d_W, 8 * w.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     double* outs[3] = {W_ww, W_ee, W_ew};
     for (int b = 0; b < nplanes / 3; ++b) {
@@ -946,7 +959,9 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
     }
     if (iters_out) {
         std::vector<int32_t> h((size_t)nsteps * c->nslot);
-        HIPCHK(c, hipMemcpy(h.data(), c->d_iters, 4 * h.size(), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->d_iters, 4 * h.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int32_t s = 0; s < nsteps; ++s)
             for (int64_t k = 0; k < c->nslot; ++k)
                 if (c->slot_col[k] >= 0)
@@ -974,7 +989,9 @@ int clm_totals(clm_ctx* c, double* water, double* energy) {
     if (r) return r;
     // zf in CSR order from the packed copy
     std::vector<double> zf((size_t)c->npface);
-    HIPCHK(c, hipMemcpy(zf.data(), c->d_zf, 8 * zf.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(zf.data(), c->d_zf, 8 * zf.size(), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int64_t s = 0; s < c->nslot; ++s) {
         int32_t col = c->slot_col[s];
         if (col < 0) continue;
