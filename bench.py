@@ -53,11 +53,11 @@ def cpu_baseline(batch, nsteps: int, ncol_sample: int, threads: int):
     return ncol_sample * nsteps / el, el
 
 
-def run_form(batch, jac, steps, warmup):
+def run_form(batch, jac, steps, warmup, solver=0):
     import torch
     from clmsoil.model import LandSimulation, model_from_batch
     w = batch.workload
-    m = model_from_batch(batch, jac_precision=jac)
+    m = model_from_batch(batch, jac_precision=jac, solver=solver)
     sim = LandSimulation(m, batch.copy_state(), w.dt, max_iters=w.newton_max_iters,
                          tol=w.newton_tol)
     sim.set_forcing_table(batch.top_water, batch.top_heat)
@@ -91,6 +91,7 @@ def main():
     ap.add_argument("--ncol", type=int, default=None, help="prefix of the workload (testing)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solver", type=int, default=0, help="0 auto, 1 group, 2 lane")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; columns shard across GPUs with no "
@@ -110,7 +111,7 @@ def main():
     for name, jac in (("fp64", 64), ("mixed", 32)):
         if batch.model == 0 and name == "mixed" and args.form == "fp64":
             continue
-        r = run_form(batch_steps, jac, steps, args.warmup)
+        r = run_form(batch_steps, jac, steps, args.warmup, args.solver)
         res[name] = r
         st = r["stats"]
         print(f"[bench] {name}: {batch.ncol * steps / r['wall']:.4g} column-steps/s, "

@@ -134,7 +134,7 @@ class _SoilModel:
 
     def __init__(self, *, parameters, domain: ColumnBatchDomain, boundary_conditions: dict,
                  sources: tuple = (), jac_precision: int = 64, newton_max_iters: int = 3,
-                 newton_tol: float = 0.0):
+                 newton_tol: float = 0.0, solver: int = L.SOLVER_AUTO):
         self.parameters = parameters
         self.domain = domain
         self.boundary_conditions = boundary_conditions
@@ -146,7 +146,8 @@ class _SoilModel:
         self._params = np.ascontiguousarray(self._param_rows(), dtype=np.float64)
         cfg = L.Config(self.model_id, self._bottom_kind(), domain.ncol,
                        L.iptr(self._col_ptr), L.dptr(self._z_face), L.dptr(self._params),
-                       self._params.shape[0], jac_precision, newton_max_iters, newton_tol)
+                       self._params.shape[0], jac_precision, newton_max_iters, newton_tol,
+                       solver)
         L.check(self._lib, None, self._lib.clm_create(C.byref(self._ctx), C.byref(cfg), None))
         self.jac_precision = jac_precision
         self.set_forcing()
@@ -431,7 +432,7 @@ def _column_norm(m: _SoilModel, dx: dict, x: dict) -> np.ndarray:
 
 
 def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | None = None,
-                     tol: float | None = None, slot: int = 0):
+                     tol: float | None = None, slot: int = 0, solver: int = L.SOLVER_AUTO):
     """Build the model a synthetic ColumnBatch describes."""
     w = batch.workload
     P = batch.params
@@ -439,7 +440,7 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
     dom = ColumnBatchDomain(batch.col_ptr, batch.z_face)
     kw = dict(jac_precision=jac_precision or w.jac_precision,
               newton_max_iters=max_iters or w.newton_max_iters,
-              newton_tol=w.newton_tol if tol is None else tol)
+              newton_tol=w.newton_tol if tol is None else tol, solver=solver)
     if batch.model == L.MODEL_RICHARDS:
         par = RichardsParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R])
         bcs = {"top": WaterFluxBC(batch.top_water[slot]), "bottom": FreeDrainage()}

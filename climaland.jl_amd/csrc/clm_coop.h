@@ -1,0 +1,440 @@
+// clm_coop.h -- group-cooperative step kernels: G lanes own one column.
+//
+// A wavefront holds 64/G columns; lane l of a group owns rows R*l .. R*l+R-1
+// of its column (level 0 = bottom).  The whole ARS111 step -- explicit stage
+// and every Newton iteration -- runs out of registers: the column's state
+// is read from HBM once and written once per step.  Neighbour values move
+// with intra-group shuffles.  Newton iterations are group-uniform, so a
+// column that needs many iterations holds back only its own wavefront.
+//
+// Linear solve.  The modified-Picard block W = dtγ J - I with
+// J = diag(1/Δz_c) M diag(D) (M: the symmetric face-conductance Laplacian,
+// D: dψ/dϑ or 1/ρc_s) is solved in the equivalent form
+//     (dtγ M - diag(Δz_c / D)) y = Δz_c ∘ R,     Δ = y / D,
+// exactly the same linear system with rows scaled by Δz_c and unknowns by
+// D.  That matrix is symmetric and strictly diagonally dominant, so
+// parallel cyclic reduction (log2 G shuffle steps; one local cyclic-
+// reduction level first when R = 2) is stable without pivoting.  For
+// EnergyHydrology the ρe-block unknown y_ρ = Δρe / ρc_s is exactly the
+// scaled quantity of the convergence norm.
+#pragma once
+#include "clm_eh.h"
+
+namespace clm {
+
+// Every lane of a group must execute each shuffle: a lane masked off by
+// divergent control flow reads back as zero to the lane that asks for it.
+template <int G, typename T>
+__device__ __forceinline__ T g_up(T v, int d) {  // value of lane l-d in the group
+    return __shfl_up(v, (unsigned)d, G);
+}
+template <int G, typename T>
+__device__ __forceinline__ T g_down(T v, int d) {  // value of lane l+d in the group
+    return __shfl_down(v, (unsigned)d, G);
+}
+template <int G, typename T>
+__device__ __forceinline__ T g_bcast(T v, int src) {
+    return __shfl(v, src, G);
+}
+template <int G>
+__device__ __forceinline__ double g_max(double v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, G));
+    return v;
+}
+
+// Parallel cyclic reduction, one row per lane (rows >= N are identity rows
+// a = c = d = 0, b = 1).  Returns x with a x_{l-1} + b x_l + c x_{l+1} = d.
+template <int G, typename JT>
+__device__ __forceinline__ JT pcr1(int gl, JT a, JT b, JT c, JT d) {
+#pragma unroll
+    for (int s = 1; s < G; s <<= 1) {
+        JT ib = (JT)1 / b;
+        JT am = g_up<G>(a, s), cm = g_up<G>(c, s), dm = g_up<G>(d, s), ibm = g_up<G>(ib, s);
+        JT ap = g_down<G>(a, s), cp = g_down<G>(c, s), dp = g_down<G>(d, s), ibp = g_down<G>(ib, s);
+        if (gl < s) { am = 0; cm = 0; dm = 0; ibm = 1; }
+        if (gl + s >= G) { ap = 0; cp = 0; dp = 0; ibp = 1; }
+        JT al = -a * ibm, ga = -c * ibp;
+        JT na = al * am, nc = ga * cp;
+        b = b + al * cm + ga * ap;
+        d = d + al * dm + ga * dp;
+        a = na;
+        c = nc;
+    }
+    return d / b;
+}
+
+// Two rows per lane (rows 2l, 2l+1): one cyclic-reduction level folds the
+// even rows into the odd ones, PCR solves the odd rows across lanes, the
+// even rows are recovered by back substitution.
+template <int G, typename JT>
+__device__ __forceinline__ void pcr2(int gl, const JT* a, const JT* b, const JT* c, const JT* d,
+                                     JT* x) {
+    // even row of the next lane (row 2l+2)
+    JT a2 = g_down<G>(a[0], 1), b2 = g_down<G>(b[0], 1), c2 = g_down<G>(c[0], 1);
+    JT d2 = g_down<G>(d[0], 1);
+    if (gl + 1 >= G) { a2 = 0; b2 = 1; c2 = 0; d2 = 0; }
+    JT al = -a[1] / b[0], ga = -c[1] / b2;
+    JT oa = al * a[0];
+    JT oc = ga * c2;
+    JT ob = b[1] + al * c[0] + ga * a2;
+    JT od = d[1] + al * d[0] + ga * d2;
+    JT xo = pcr1<G, JT>(gl, oa, ob, oc, od);
+    JT xm = g_up<G>(xo, 1);  // x_{2l-1}
+    if (gl == 0) xm = 0;
+    x[1] = xo;
+    x[0] = (d[0] - a[0] * xm - c[0] * xo) / b[0];
+}
+
+template <int G, int R, typename JT>
+__device__ __forceinline__ void pcr_solve(int gl, JT (&a)[R], JT (&b)[R], JT (&c)[R], JT (&d)[R],
+                                          JT (&x)[R]) {
+    if constexpr (R == 1) {
+        x[0] = pcr1<G, JT>(gl, a[0], b[0], c[0], d[0]);
+    } else {
+        pcr2<G, JT>(gl, a, b, c, d, x);
+    }
+}
+
+// neighbour values across the lane boundary: the row below my first row is
+// the previous lane's last row, the row above my last row the next lane's
+// first row
+template <int G, int R>
+__device__ __forceinline__ double below(const double (&v)[R], int k) {
+    return k > 0 ? v[k - 1] : g_up<G>(v[R - 1], 1);
+}
+
+struct GroupId {
+    int gl, grp;
+    bool live;
+};
+
+template <int G>
+__device__ __forceinline__ GroupId group_id(int ncols_bin) {
+    constexpr int GPW = 64 / G;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int lane = threadIdx.x & 63;
+    GroupId id;
+    id.gl = lane % G;
+    id.grp = (t >> 6) * GPW + lane / G;
+    id.live = id.grp < ncols_bin;
+    return id;
+}
+
+// ---- RichardsModel ----------------------------------------------------------
+template <int G, int R, typename JT>
+__global__ __launch_bounds__(256) void k_rre_coop(
+    const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
+    const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
+    double* __restrict__ xg, double* __restrict__ iw, const double* __restrict__ f_top,
+    const double* __restrict__ f_bot, int bc_bottom, double dt, int max_iters, double tol,
+    int32_t* __restrict__ iters) {
+    GroupId id = group_id<G>(ncols_bin);
+    if (!id.live) return;
+    const int gl = id.gl;
+    const int col = bin_cols[id.grp];
+    const int64_t base = col_ptr[col];
+    const int N = (int)(col_ptr[col + 1] - base);
+    const int64_t fbase = base + col;
+    const VG v = make_vg(par[P_VG_ALPHA * ncol + col], par[P_VG_N * ncol + col],
+                         par[P_THETA_R * ncol + col], par[P_SS * ncol + col],
+                         par[P_KSAT * ncol + col]);
+    const double nu = par[P_NU * ncol + col];
+    const double top = f_top[col];
+
+    double x[R], u0[R], zc[R], dzc[R], dzf[R];
+    bool on[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int r = gl * R + k;
+        on[k] = r < N;
+        double z0 = 0.0, z1 = 1.0, xv = 0.0;
+        if (on[k]) {
+            xv = xg[base + r];
+            z0 = zf[fbase + r];
+            z1 = zf[fbase + r + 1];
+        }
+        x[k] = xv;
+        u0[k] = xv;  // explicit tendency is zero (rre.jl:214-243): U0 = u_n
+        zc[k] = 0.5 * (z0 + z1);
+        dzc[k] = z1 - z0;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);  // face below row
+
+    double bot = 0.0;
+    int kit = 0;
+    bool conv = false;
+    double err = 0.0;
+    for (kit = 1; kit <= max_iters; ++kit) {
+        double K[R], h[R], D[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            PsiD p = psi_dpsi(v, nu, x[k]);
+            K[k] = on[k] ? hyd_cond_from_a(v, p.S, p.a) : 0.0;
+            h[k] = p.psi + zc[k];
+            D[k] = p.D;
+        }
+        if (kit == 1)  // lagged boundary fluxes from update_cache!(u_n)
+            bot = (bc_bottom == BC_FREE_DRAINAGE) ? -1.0 * g_bcast<G>(K[0], 0) : f_bot[col];
+        // face below each row: conductance w and flux F
+        double w[R], F[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            int r = gl * R + k;
+            double Km = below<G, R>(K, k), hm = below<G, R>(h, k);
+            double ww = 0.5 * (Km + K[k]) / dzf[k];
+            w[k] = (r == 0) ? 0.0 : ww;
+            F[k] = (r == 0) ? bot : -ww * (h[k] - hm);
+        }
+        JT a[R], b[R], c[R], d[R], y[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            int r = gl * R + k;
+            double wu = (k + 1 < R) ? w[k + 1] : g_down<G>(w[0], 1);
+            double Fu = (k + 1 < R) ? F[k + 1] : g_down<G>(F[0], 1);
+            if (r >= N - 1) {
+                wu = 0.0;
+                Fu = top;
+            }
+            // Δz_c R = Δz_c (U0 - x) - Δt (F_up - F_down)
+            double rhs = dzc[k] * (u0[k] - x[k]) - dt * (Fu - F[k]);
+            a[k] = (JT)(dt * w[k]);
+            c[k] = (JT)(dt * wu);
+            b[k] = (JT)(-dt * (w[k] + wu) - dzc[k] / D[k]);
+            d[k] = (JT)rhs;
+            if (!on[k]) {
+                a[k] = 0;
+                b[k] = 1;
+                c[k] = 0;
+                d[k] = 0;
+            }
+        }
+        pcr_solve<G, R, JT>(gl, a, b, c, d, y);
+        double e = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            double dx = on[k] ? (double)y[k] / D[k] : 0.0;
+            x[k] -= dx;
+            e = nan_max(e, fabs(dx));
+        }
+        err = g_max<G>(e);
+        conv = (tol > 0.0) && (err < tol);
+        if (conv || !isfinite(err)) break;
+    }
+    if (kit > max_iters) kit = max_iters;
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+        if (on[k]) xg[base + gl * R + k] = x[k];
+    if (gl == 0) {
+        iw[col] = iw[col] + dt * (-(top - bot));  // ∫F: -I block, exact
+        int32_t flag = kit;
+        if (tol > 0.0 && !conv) flag |= 0x10000;
+        if (!isfinite(err)) flag |= 0x20000;
+        iters[col] = flag;
+    }
+}
+
+// ---- EnergyHydrology ----------------------------------------------------------
+template <int G, int R, typename JT>
+__global__ __launch_bounds__(256) void k_eh_coop(
+    const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
+    const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
+    double* __restrict__ xvg, double* __restrict__ xig, double* __restrict__ xeg,
+    double* __restrict__ iw, double* __restrict__ ie, const double* __restrict__ ftw,
+    const double* __restrict__ fth, const double* __restrict__ fbw,
+    const double* __restrict__ fbh, int bc_bottom, double dt, int max_iters, double tol,
+    int32_t* __restrict__ iters) {
+    GroupId id = group_id<G>(ncols_bin);
+    if (!id.live) return;
+    const int gl = id.gl;
+    const int col = bin_cols[id.grp];
+    const int64_t base = col_ptr[col];
+    const int N = (int)(col_ptr[col + 1] - base);
+    const int64_t fbase = base + col;
+    const EHPar e = load_eh(par, ncol, col);
+    const double topw = ftw[col], toph = fth[col];
+
+    double xv[R], xi[R], xe[R], u0v[R], u0e[R], zc[R], dzc[R], dzf[R];
+    double Kl[R], kl[R], rcs[R], Tp[R], Pp[R];
+    bool on[R];
+    int bad = 0;
+    // ---- stage 1: update_cache!(u_n) + phase change -> U0 -------------------
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int r = gl * R + k;
+        on[k] = r < N;
+        double z0 = 0.0, z1 = 1.0, v = 0.3, t = 0.0, re = 0.0;
+        if (on[k]) {
+            v = xvg[base + r];
+            t = xig[base + r];
+            re = xeg[base + r];
+            z0 = zf[fbase + r];
+            z1 = zf[fbase + r + 1];
+        }
+        zc[k] = 0.5 * (z0 + z1);
+        dzc[k] = z1 - z0;
+        EHAux a = eh_cell_aux(e, v, t, re);
+        double F = 0.0;
+        if (on[k]) {
+            double tau = 3.0 * heat_capacity(a.thl, t, e.rhoc_ds) * dzc[k] * dzc[k] / a.kap;
+            F = phase_change(e.v, e.nu, a.thl, t, a.T, tau, &bad);
+        }
+        u0v[k] = v + dt * (-F);
+        xi[k] = t + dt * ((RHO_L / RHO_I) * F);
+        u0e[k] = re;
+        xv[k] = u0v[k];
+        xe[k] = re;
+        Kl[k] = on[k] ? a.K : 0.0;
+        kl[k] = on[k] ? a.kap : 0.0;
+        rcs[k] = heat_capacity(a.thl, xi[k], e.rhoc_ds);  // 1/D of the ρe block (lagged θ_l)
+        Tp[k] = a.T;
+        Pp[k] = a.psi;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);
+    double botw, both;
+    {
+        double K0 = g_bcast<G>(Kl[0], 0), T0 = g_bcast<G>(Tp[0], 0);
+        if (bc_bottom == BC_FREE_DRAINAGE) {  // boundary_conditions.jl:590-608
+            botw = -1.0 * K0;
+            both = -1.0 * K0 * energy_liq(T0);
+        } else {
+            botw = fbw[col];
+            both = fbh[col];
+        }
+    }
+    // conductances of the face below each row (lagged K, κ)
+    double wK[R], wk[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int r = gl * R + k;
+        double Km = below<G, R>(Kl, k), km = below<G, R>(kl, k);
+        wK[k] = (r == 0) ? 0.0 : 0.5 * (Km + Kl[k]) / dzf[k];
+        wk[k] = (r == 0) ? 0.0 : 0.5 * (km + kl[k]) / dzf[k];
+    }
+
+    int kit;
+    bool conv = false;
+    double err = 0.0;
+    for (kit = 1; kit <= max_iters; ++kit) {
+        double D[R], h[R], E[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            D[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).D;  // Wfact(x): dψ/dϑ at the iterate
+            h[k] = Pp[k] + zc[k];                          // p.ψ
+            E[k] = energy_liq(Tp[k]) * Kl[k];              // p.T
+        }
+        // face below each row: Darcy flux F, heat flux G, advective conductance wE
+        double F[R], Gf[R], wE[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            int r = gl * R + k;
+            double hm = below<G, R>(h, k), Tm = below<G, R>(Tp, k), Em = below<G, R>(E, k);
+            double gh = (h[k] - hm) / dzf[k];
+            double eK = 0.5 * (Em + E[k]);
+            wE[k] = (r == 0) ? 0.0 : eK / dzf[k];
+            F[k] = (r == 0) ? botw : -(wK[k] * dzf[k]) * gh;
+            Gf[k] = (r == 0) ? both : -(wk[k] * dzf[k]) * ((Tp[k] - Tm) / dzf[k]) - eK * gh;
+        }
+        JT a[R], b[R], c[R], d[R], y[R];
+        double dRe[R];  // Δz_c R_ρe
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            int r = gl * R + k;
+            bool last = r >= N - 1;
+            // shuffles run on every lane of the group (a lane that skips one
+            // would read as zero to its neighbour); boundary selects come after
+            double wKu = (k + 1 < R) ? wK[k + 1] : g_down<G>(wK[0], 1);
+            double Fu = (k + 1 < R) ? F[k + 1] : g_down<G>(F[0], 1);
+            double Gu = (k + 1 < R) ? Gf[k + 1] : g_down<G>(Gf[0], 1);
+            double wu = last ? 0.0 : wKu;
+            if (last) {
+                Fu = topw;
+                Gu = toph;
+            }
+            double rv = dzc[k] * (u0v[k] - xv[k]) - dt * (Fu - F[k]);
+            dRe[k] = dzc[k] * (u0e[k] - xe[k]) - dt * (Gu - Gf[k]);
+            a[k] = (JT)(dt * wK[k]);
+            c[k] = (JT)(dt * wu);
+            b[k] = (JT)(-dt * (wK[k] + wu) - dzc[k] / D[k]);
+            d[k] = (JT)rv;
+            if (!on[k]) {
+                a[k] = 0;
+                b[k] = 1;
+                c[k] = 0;
+                d[k] = 0;
+            }
+        }
+        pcr_solve<G, R, JT>(gl, a, b, c, d, y);  // y = D ∘ Δϑ
+        // ρe block right-hand side: Δz_c (R_ρe - W_ρϑ Δϑ), with
+        // Δz_c W_ρϑ Δϑ = dt M_E y - Δz_c Δϑ (the reference's "- I", :554-556)
+        double yv[R], dv[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            yv[k] = on[k] ? (double)y[k] : 0.0;
+            dv[k] = on[k] ? yv[k] / D[k] : 0.0;
+        }
+        double e2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            int r = gl * R + k;
+            bool last = r >= N - 1;
+            double wEn = (k + 1 < R) ? wE[k + 1] : g_down<G>(wE[0], 1);
+            double wkn = (k + 1 < R) ? wk[k + 1] : g_down<G>(wk[0], 1);
+            double ym = below<G, R>(yv, k);
+            double yp = (k + 1 < R) ? yv[k + 1] : g_down<G>(yv[0], 1);
+            double wEu = last ? 0.0 : wEn;
+            double wku = last ? 0.0 : wkn;
+            double MEy = wE[k] * (ym - yv[k]) + wEu * (yp - yv[k]);
+            double rhs = dRe[k] - (dt * MEy - dzc[k] * dv[k]);
+            a[k] = (JT)(dt * wk[k]);
+            c[k] = (JT)(dt * wku);
+            b[k] = (JT)(-dt * (wk[k] + wku) - dzc[k] * rcs[k]);
+            d[k] = (JT)rhs;
+            if (!on[k]) {
+                a[k] = 0;
+                b[k] = 1;
+                c[k] = 0;
+                d[k] = 0;
+            }
+            xv[k] -= dv[k];
+            e2 = nan_max(e2, fabs(dv[k]));
+        }
+        pcr_solve<G, R, JT>(gl, a, b, c, d, y);  // y = Δρe / ρc_s
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            double yr = on[k] ? (double)y[k] : 0.0;
+            xe[k] -= yr * rcs[k];
+            e2 = nan_max(e2, fabs(yr));
+            // cache_imp!(x): T and ψ at the new iterate (energy_hydrology.jl:427-445)
+            Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
+            Pp[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).psi;
+        }
+        err = g_max<G>(e2);
+        conv = (tol > 0.0) && (err < tol);
+        if (conv || !isfinite(err)) break;
+    }
+    if (kit > max_iters) kit = max_iters;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        if (on[k]) {
+            int64_t i = base + gl * R + k;
+            xvg[i] = xv[k];
+            xig[i] = xi[k];
+            xeg[i] = xe[k];
+        }
+    }
+    int bad_g = (int)g_max<G>((double)bad);
+    if (gl == 0) {
+        iw[col] = iw[col] + dt * (-(topw - botw));
+        ie[col] = ie[col] + dt * (-(toph - both));
+        int32_t flag = kit;
+        if (tol > 0.0 && !conv) flag |= 0x10000;
+        if (!isfinite(err)) flag |= 0x20000;
+        if (bad_g) flag |= 0x40000;
+        iters[col] = flag;
+    }
+}
+
+}  // namespace clm

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/clm_soil.h"
+#include "clm_coop.h"
 #include "clm_eh.h"
 #include "clm_layout.h"
 #include "clm_rre.h"
@@ -69,6 +70,18 @@ struct clm_ctx {
     int32_t* d_iters = nullptr;
     int64_t iters_cap = 0;  // in steps
     DevStats* d_stats = nullptr;
+    // group-cooperative path: CSR state in column order, columns binned by depth
+    int solver = CLM_SOLVER_GROUP;
+    int32_t* d_col_ptr = nullptr;
+    double *d_cpar = nullptr, *d_czf = nullptr;
+    double *d_cv = nullptr, *d_ci = nullptr, *d_ce = nullptr, *d_ciw = nullptr, *d_cie = nullptr;
+    double *d_cftw = nullptr, *d_cfth = nullptr, *d_cfbw = nullptr, *d_cfbh = nullptr;
+    struct Bin {
+        int G, R;
+        int32_t n;
+        int32_t* d_cols;
+    };
+    std::vector<Bin> bins;
     // stream, graph, events
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -128,6 +141,18 @@ void dfree(T*& p) {
 }
 
 bool is_eh(const clm_ctx* c) { return c->model == CLM_MODEL_ENERGY_HYDROLOGY; }
+bool is_group(const clm_ctx* c) { return c->solver == CLM_SOLVER_GROUP; }
+// length of per-column device vectors of the coarse path
+int64_t ncolvec(const clm_ctx* c) { return is_group(c) ? c->ncol : c->nslot; }
+
+// group-solver bins: (lanes per column G, rows per lane R) for a column of N layers
+void bin_shape(int N, int* G, int* R) {
+    if (N <= 8) { *G = 8; *R = 1; }
+    else if (N <= 16) { *G = 16; *R = 1; }
+    else if (N <= 32) { *G = 32; *R = 1; }
+    else if (N <= 64) { *G = 64; *R = 1; }
+    else { *G = 64; *R = 2; }
+}
 size_t jt_size(const clm_ctx* c) { return c->jac_prec == 32 ? sizeof(float) : sizeof(double); }
 
 Layout layout(const clm_ctx* c) {
@@ -258,7 +283,7 @@ __global__ __launch_bounds__(256) void k_reduce_stats(const int32_t* __restrict_
     unsigned int mx = 0;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrec;
          r += (int64_t)gridDim.x * blockDim.x) {
-        if (nl[r % nslot] <= 0) continue;
+        if (nl && nl[r % nslot] <= 0) continue;
         int32_t v = it[r];
         unsigned int k = (unsigned int)(v & 0xffff);
         sum += k;
@@ -290,10 +315,11 @@ __global__ __launch_bounds__(256) void k_reduce_stats(const int32_t* __restrict_
 
 int collect_stats(clm_ctx* c, int64_t nsteps, clm_step_stats* out) {
     hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, c->d_stats);
-    int64_t nrec = nsteps * c->nslot;
+    int64_t nv = ncolvec(c);
+    int64_t nrec = nsteps * nv;
     unsigned g = (unsigned)std::min<int64_t>((nrec + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_reduce_stats, dim3(g), dim3(256), 0, c->stream, c->d_iters, c->d_nl,
-                       c->nslot, nrec, c->d_stats);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(g), dim3(256), 0, c->stream, c->d_iters,
+                       is_group(c) ? nullptr : c->d_nl, nv, nrec, c->d_stats);
     HIPCHK(c, hipGetLastError());
     DevStats h;
     HIPCHK(c, hipMemcpyAsync(&h, c->d_stats, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -310,7 +336,7 @@ int collect_stats(clm_ctx* c, int64_t nsteps, clm_step_stats* out) {
 int ensure_iters(clm_ctx* c, int64_t nsteps) {
     if (nsteps <= c->iters_cap) return CLM_OK;
     dfree(c->d_iters);
-    int r = dalloc(c, &c->d_iters, nsteps * c->nslot);
+    int r = dalloc(c, &c->d_iters, nsteps * ncolvec(c));
     if (r) return r;
     c->iters_cap = nsteps;
     return CLM_OK;
@@ -347,11 +373,68 @@ void launch_step_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
     }
 }
 
+// one launch per depth bin; every kernel is independent of the others
+template <typename JT, int G, int R>
+void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const double* fth,
+                double dt, int32_t* iters) {
+    constexpr int gpb = 256 / G;  // columns per 256-thread block
+    dim3 g((unsigned)((bn.n + gpb - 1) / gpb)), b(256);
+    if (!is_eh(c)) {
+        hipLaunchKernelGGL((k_rre_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
+                           bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ciw, ftw, c->d_cfbw,
+                           c->bc_bottom, dt, c->max_iters, c->tol, iters);
+    } else {
+        hipLaunchKernelGGL((k_eh_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
+                           bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci, c->d_ce,
+                           c->d_ciw, c->d_cie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt,
+                           c->max_iters, c->tol, iters);
+    }
+}
+
+template <typename JT>
+void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
+                    int32_t* iters) {
+    for (const auto& bn : c->bins) {
+        if (bn.n == 0) continue;
+        if (bn.G == 8) launch_bin<JT, 8, 1>(c, bn, ftw, fth, dt, iters);
+        else if (bn.G == 16) launch_bin<JT, 16, 1>(c, bn, ftw, fth, dt, iters);
+        else if (bn.G == 32) launch_bin<JT, 32, 1>(c, bn, ftw, fth, dt, iters);
+        else if (bn.R == 1) launch_bin<JT, 64, 1>(c, bn, ftw, fth, dt, iters);
+        else launch_bin<JT, 64, 2>(c, bn, ftw, fth, dt, iters);
+    }
+}
+
 void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters) {
+    if (is_group(c)) {
+        if (c->jac_prec == 32)
+            launch_group_t<float>(c, ftw, fth, dt, iters);
+        else
+            launch_group_t<double>(c, ftw, fth, dt, iters);
+        return;
+    }
     if (c->jac_prec == 32)
         launch_step_t<float>(c, ftw, fth, dt, iters);
     else
         launch_step_t<double>(c, ftw, fth, dt, iters);
+}
+
+int upload_raw(clm_ctx* c, const double* h, double* d, int64_t n) {
+    if (!h) {
+        HIPCHK(c, hipMemsetAsync(d, 0, sizeof(double) * (size_t)n, c->stream));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int download_raw(clm_ctx* c, const double* d, double* h, int64_t n) {
+    if (!h) return CLM_OK;
+    HIPCHK(c, hipMemcpyAsync(h, d, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
 }
 
 void destroy_graph(clm_ctx* c) {
@@ -390,6 +473,8 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
         return set_err(nullptr, CLM_E_ARG, "jac_precision must be 64 or 32");
     if (cfg->newton_max_iters < 1 || cfg->newton_max_iters > 0xffff)
         return set_err(nullptr, CLM_E_ARG, "newton_max_iters out of range");
+    if (cfg->solver < CLM_SOLVER_AUTO || cfg->solver > CLM_SOLVER_LANE)
+        return set_err(nullptr, CLM_E_ARG, "unknown solver %d", cfg->solver);
     if (cfg->col_ptr[0] != 0) return set_err(nullptr, CLM_E_ARG, "col_ptr[0] must be 0");
     for (int64_t i = 0; i < cfg->ncol; ++i) {
         int32_t n = cfg->col_ptr[i + 1] - cfg->col_ptr[i];
@@ -405,7 +490,16 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
                                (long long)i);
     }
 
+    int32_t nmax_all = 0;
+    for (int64_t i = 0; i < cfg->ncol; ++i)
+        nmax_all = std::max(nmax_all, cfg->col_ptr[i + 1] - cfg->col_ptr[i]);
+    int solver = cfg->solver;
+    if (solver == CLM_SOLVER_AUTO) solver = nmax_all <= 128 ? CLM_SOLVER_GROUP : CLM_SOLVER_LANE;
+    if (solver == CLM_SOLVER_GROUP && nmax_all > 128)
+        return set_err(nullptr, CLM_E_ARG, "group solver takes columns of up to 128 layers");
+
     clm_ctx* c = new clm_ctx();
+    c->solver = solver;
     c->model = cfg->model;
     c->bc_bottom = cfg->bc_bottom;
     c->jac_prec = cfg->jac_precision;
@@ -502,6 +596,44 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
         ALLOC(c->d_Pp, c->npcell);
         ALLOC(c->d_Re, c->npcell);
     }
+    if (is_group(c)) {
+        ALLOC(c->d_col_ptr, c->ncol + 1);
+        ALLOC(c->d_cpar, (int64_t)NPARAM_EH * c->ncol);
+        ALLOC(c->d_czf, c->ncell + c->ncol);
+        ALLOC(c->d_cv, c->ncell);
+        ALLOC(c->d_ciw, c->ncol);
+        ALLOC(c->d_cftw, c->ncol);
+        ALLOC(c->d_cfth, c->ncol);
+        ALLOC(c->d_cfbw, c->ncol);
+        ALLOC(c->d_cfbh, c->ncol);
+        if (is_eh(c)) {
+            ALLOC(c->d_ci, c->ncell);
+            ALLOC(c->d_ce, c->ncell);
+            ALLOC(c->d_cie, c->ncol);
+        }
+        // depth bins
+        const int shapes[5][2] = {{8, 1}, {16, 1}, {32, 1}, {64, 1}, {64, 2}};
+        std::vector<std::vector<int32_t>> cols(5);
+        for (int64_t i = 0; i < c->ncol; ++i) {
+            int G, R;
+            bin_shape(c->col_ptr[i + 1] - c->col_ptr[i], &G, &R);
+            for (int k = 0; k < 5; ++k)
+                if (shapes[k][0] == G && shapes[k][1] == R) cols[k].push_back((int32_t)i);
+        }
+        for (int k = 0; k < 5; ++k) {
+            clm_ctx::Bin bn{shapes[k][0], shapes[k][1], (int32_t)cols[k].size(), nullptr};
+            if (bn.n > 0) {
+                ALLOC(bn.d_cols, bn.n);
+                if (hipMemcpyAsync(bn.d_cols, cols[k].data(), 4 * cols[k].size(),
+                                   hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                    hipStreamSynchronize(c->stream) != hipSuccess) {
+                    set_err(c, CLM_E_HIP, "bin upload failed");
+                    return fail(CLM_E_HIP);
+                }
+            }
+            c->bins.push_back(bn);
+        }
+    }
 #undef ALLOC
     {
         size_t wbytes = jt_size(c) * (size_t)c->npcell * (is_eh(c) ? 9 : 3);
@@ -542,6 +674,17 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
         up(c->d_slot_col, c->slot_col.data(), 4 * c->slot_col.size());
         up(c->d_par, par.data(), 8 * par.size());
         up(c->d_zf, zf.data(), 8 * zf.size());
+        if (is_group(c)) {
+            std::vector<double> cpar((size_t)NPARAM_EH * c->ncol, 1.0);
+            for (int p = 0; p < need; ++p)
+                std::memcpy(cpar.data() + (size_t)p * c->ncol, cfg->params + (size_t)p * c->ncol,
+                            8 * (size_t)c->ncol);
+            std::vector<double> czf(cfg->z_face, cfg->z_face + c->ncell + c->ncol);
+            up(c->d_col_ptr, c->col_ptr.data(), 4 * c->col_ptr.size());
+            up(c->d_cpar, cpar.data(), 8 * cpar.size());
+            up(c->d_czf, czf.data(), 8 * czf.size());
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) {
             set_err(c, CLM_E_HIP, "upload failed: %s", hipGetErrorString(e));
@@ -603,6 +746,19 @@ int clm_destroy(clm_ctx* c) {
     dfree(c->d_scr);
     dfree(c->d_iters);
     dfree(c->d_stats);
+    dfree(c->d_col_ptr);
+    dfree(c->d_cpar);
+    dfree(c->d_czf);
+    dfree(c->d_cv);
+    dfree(c->d_ci);
+    dfree(c->d_ce);
+    dfree(c->d_ciw);
+    dfree(c->d_cie);
+    dfree(c->d_cftw);
+    dfree(c->d_cfth);
+    dfree(c->d_cfbw);
+    dfree(c->d_cfbh);
+    for (auto& bn : c->bins) dfree(bn.d_cols);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
@@ -612,12 +768,37 @@ int clm_destroy(clm_ctx* c) {
 
 int clm_set_state(clm_ctx* c, const clm_state* Y) {
     if (!c) return CLM_E_ARG;
+    if (is_group(c)) {
+        if (!Y || !Y->vartheta_l) return set_err(c, CLM_E_ARG, "state: vartheta_l required");
+        int r;
+        if ((r = upload_raw(c, Y->vartheta_l, c->d_cv, c->ncell))) return r;
+        if ((r = upload_raw(c, Y->intF_water, c->d_ciw, c->ncol))) return r;
+        if (is_eh(c)) {
+            if (!Y->theta_i || !Y->rho_e_int)
+                return set_err(c, CLM_E_ARG, "EnergyHydrology state needs theta_i and rho_e_int");
+            if ((r = upload_raw(c, Y->theta_i, c->d_ci, c->ncell))) return r;
+            if ((r = upload_raw(c, Y->rho_e_int, c->d_ce, c->ncell))) return r;
+            if ((r = upload_raw(c, Y->intF_energy, c->d_cie, c->ncol))) return r;
+        }
+        return CLM_OK;
+    }
     double* dst[5] = {c->d_xv, c->d_xi, c->d_xe, c->d_iw, c->d_ie};
     return upload_state(c, Y, dst);
 }
 
 int clm_get_state(clm_ctx* c, clm_state* Y) {
     if (!c || !Y) return CLM_E_ARG;
+    if (is_group(c)) {
+        int r;
+        if ((r = download_raw(c, c->d_cv, Y->vartheta_l, c->ncell))) return r;
+        if ((r = download_raw(c, c->d_ciw, Y->intF_water, c->ncol))) return r;
+        if (is_eh(c)) {
+            if ((r = download_raw(c, c->d_ci, Y->theta_i, c->ncell))) return r;
+            if ((r = download_raw(c, c->d_ce, Y->rho_e_int, c->ncell))) return r;
+            if ((r = download_raw(c, c->d_cie, Y->intF_energy, c->ncol))) return r;
+        }
+        return CLM_OK;
+    }
     double* src[5] = {c->d_xv, c->d_xi, c->d_xe, c->d_iw, c->d_ie};
     return download_state(c, src, Y);
 }
@@ -629,7 +810,14 @@ int clm_set_forcing(clm_ctx* c, const double* tw, const double* th, const double
     if ((r = upload_cols(c, tw, c->d_ftw))) return r;
     if ((r = upload_cols(c, th, c->d_fth))) return r;
     if ((r = upload_cols(c, bw, c->d_fbw))) return r;
-    return upload_cols(c, bh, c->d_fbh);
+    if ((r = upload_cols(c, bh, c->d_fbh))) return r;
+    if (is_group(c)) {
+        if ((r = upload_raw(c, tw, c->d_cftw, c->ncol))) return r;
+        if ((r = upload_raw(c, th, c->d_cfth, c->ncol))) return r;
+        if ((r = upload_raw(c, bw, c->d_cfbw, c->ncol))) return r;
+        if ((r = upload_raw(c, bh, c->d_cfbh, c->ncol))) return r;
+    }
+    return CLM_OK;
 }
 
 int clm_set_forcing_table(clm_ctx* c, int32_t nslots, const double* tw, const double* th) {
@@ -639,10 +827,15 @@ int clm_set_forcing_table(clm_ctx* c, int32_t nslots, const double* tw, const do
         dfree(c->d_tab_w);
         dfree(c->d_tab_h);
         int r;
-        if ((r = dalloc(c, &c->d_tab_w, (int64_t)nslots * c->nslot))) return r;
-        if ((r = dalloc(c, &c->d_tab_h, (int64_t)nslots * c->nslot))) return r;
+        if ((r = dalloc(c, &c->d_tab_w, (int64_t)nslots * ncolvec(c)))) return r;
+        if ((r = dalloc(c, &c->d_tab_h, (int64_t)nslots * ncolvec(c)))) return r;
         c->tab_slots = nslots;
         destroy_graph(c);
+    }
+    if (is_group(c)) {
+        int r;
+        if ((r = upload_raw(c, tw, c->d_tab_w, (int64_t)nslots * c->ncol))) return r;
+        return upload_raw(c, th, c->d_tab_h, (int64_t)nslots * c->ncol);
     }
     for (int32_t k = 0; k < nslots; ++k) {
         int r;
@@ -894,7 +1087,10 @@ int clm_step(clm_ctx* c, double t, double dt, clm_step_stats* stats) {
     int r = ensure_iters(c, 1);
     if (r) return r;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    launch_step(c, c->d_ftw, c->d_fth, dt, c->d_iters);
+    if (is_group(c))
+        launch_step(c, c->d_cftw, c->d_cfth, dt, c->d_iters);
+    else
+        launch_step(c, c->d_ftw, c->d_fth, dt, c->d_iters);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->have_cache = false;  // the fused step does not maintain the fine-hook cache
@@ -929,8 +1125,9 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int32_t s = 0; s < nsteps; ++s) {
             int64_t slot = (s / steps_per_slot) % c->tab_slots;
-            launch_step(c, c->d_tab_w + slot * c->nslot, c->d_tab_h + slot * c->nslot, dt,
-                        c->d_iters + (int64_t)s * c->nslot);
+            int64_t nv = ncolvec(c);
+            launch_step(c, c->d_tab_w + slot * nv, c->d_tab_h + slot * nv, dt,
+                        c->d_iters + (int64_t)s * nv);
         }
         hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
         if (e != hipSuccess) {
@@ -957,7 +1154,11 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
         if ((r = collect_stats(c, nsteps, stats))) return r;
         stats->ms = ms;
     }
-    if (iters_out) {
+    if (iters_out && is_group(c)) {
+        HIPCHK(c, hipMemcpyAsync(iters_out, c->d_iters, 4 * (size_t)nsteps * c->ncol,
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    } else if (iters_out) {
         std::vector<int32_t> h((size_t)nsteps * c->nslot);
         HIPCHK(c, hipMemcpyAsync(h.data(), c->d_iters, 4 * h.size(), hipMemcpyDeviceToHost,
                                  c->stream));

@@ -90,6 +90,15 @@ enum {
 };
 #define CLM_NPARAM_RRE 6
 
+/* Coarse-path solvers (clm_step / clm_run).  Both solve the same
+ * block-tridiagonal Newton systems; they differ in how a column maps onto
+ * the GPU.  AUTO picks GROUP for columns of up to 128 layers, else LANE. */
+enum {
+    CLM_SOLVER_AUTO = 0,
+    CLM_SOLVER_GROUP = 1, /* 8-64 lanes per column, registers + parallel cyclic reduction */
+    CLM_SOLVER_LANE = 2   /* one column per lane, Thomas through HBM scratch */
+};
+
 typedef struct clm_ctx clm_ctx;
 
 typedef struct {
@@ -106,6 +115,8 @@ typedef struct {
     double newton_tol;        /* <= 0: exactly max_iters iterations (no checker);
                                  > 0: per-column stop when max|Δx| < tol over
                                  the cell unknowns (ρe_int scaled by 1/ρc_s) */
+    int32_t solver;           /* CLM_SOLVER_*: how clm_step / clm_run solve */
+    int32_t reserved_;
 } clm_config;
 
 /* Prognostic state Y.soil.  RRE: vartheta_l, intF_water.

@@ -41,6 +41,23 @@
 #include "clm_oracle.h"
 #include <float.h>
 #include <math.h>
+
+/* Transcendentals of the closures.  The default build evaluates them in
+ * double exactly as the reference does; -DORC_XPREC evaluates them in long
+ * double (rounded back to double) -- the same mathematics with different
+ * rounding, used by the tests to measure how strongly a computation reacts
+ * to closure rounding alone. */
+#ifdef ORC_XPREC
+#define POW(x, y) ((double)powl((long double)(x), (long double)(y)))
+#define EXP(x) ((double)expl((long double)(x)))
+#define LOG(x) ((double)logl((long double)(x)))
+#define SQRT(x) ((double)sqrtl((long double)(x)))
+#else
+#define POW(x, y) pow((x), (y))
+#define EXP(x) exp(x)
+#define LOG(x) log(x)
+#define SQRT(x) sqrt(x)
+#endif
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -68,7 +85,7 @@ static const double GAMMA_T_REF = 288.0;
 
 #define LH_F0 (LH_S0 - LH_V0) /* Parameters.jl:58 */
 
-/* sqrt(eps(Float64)) as used by the clipping in effective_saturation */
+/* SQRT(eps(Float64)) as used by the clipping in effective_saturation */
 #define SQRT_EPS 1.4901161193847656e-08
 
 void orc_constants(double* out, int n) {
@@ -97,7 +114,7 @@ double orc_volumetric_liquid_fraction(double vartheta, double nu_eff, double the
 /* :59-63, m = 1 - 1/n from retention_models.jl:40-44 */
 double orc_matric_potential(double alpha, double n, double S) {
     double m = 1.0 - 1.0 / n;
-    return -pow((pow(S, -1.0 / m) - 1.0) * pow(alpha, -n), 1.0 / n);
+    return -POW((POW(S, -1.0 / m) - 1.0) * POW(alpha, -n), 1.0 / n);
 }
 
 /* :72-77 (the reference raises for psi > 0) */
@@ -107,7 +124,7 @@ double orc_inverse_matric_potential(double alpha, double n, double psi, int* err
         return NAN;
     }
     double m = 1.0 - 1.0 / n;
-    return pow(1.0 + pow(alpha * fabs(psi), n), -m);
+    return POW(1.0 + POW(alpha * fabs(psi), n), -m);
 }
 
 /* :109-127 */
@@ -128,7 +145,7 @@ double orc_dpsidtheta(double alpha, double n, double vartheta, double nu_eff,
     double m = 1.0 - 1.0 / n;
     if (S < 1.0) {
         return 1.0 / (alpha * m * n) / (ns - theta_r) *
-               pow(pow(S, -1.0 / m) - 1.0, 1.0 / n - 1.0) * pow(S, -1.0 / m - 1.0);
+               POW(POW(S, -1.0 / m) - 1.0, 1.0 / n - 1.0) * POW(S, -1.0 / m - 1.0);
     }
     return 1.0 / S_s;
 }
@@ -138,7 +155,7 @@ double orc_hydraulic_conductivity(double n, double K_sat, double S) {
     double m = 1.0 - 1.0 / n;
     double K;
     if (S < 1.0) {
-        K = sqrt(S) * pow(1.0 - pow(1.0 - pow(S, 1.0 / m), m), 2.0);
+        K = SQRT(S) * POW(1.0 - POW(1.0 - POW(S, 1.0 / m), m), 2.0);
     } else {
         K = 1.0;
     }
@@ -146,11 +163,11 @@ double orc_hydraulic_conductivity(double n, double K_sat, double S) {
 }
 
 /* :302-305 */
-double orc_impedance_factor(double f_i, double Omega) { return pow(10.0, -Omega * f_i); }
+double orc_impedance_factor(double f_i, double Omega) { return POW(10.0, -Omega * f_i); }
 
 /* :320-324 */
 double orc_viscosity_factor(double T, double gamma, double gamma_T_ref) {
-    return exp(gamma * (T - gamma_T_ref));
+    return EXP(gamma * (T - gamma_T_ref));
 }
 
 /* ---- heat closures (soil_heat_parameterizations.jl) ------------------- */
@@ -182,7 +199,7 @@ double orc_volumetric_internal_energy_liq(double T) {
 double orc_kappa_sat(double theta_l, double theta_i, double k_unfrozen, double k_frozen) {
     double tw = theta_l + theta_i;
     if (tw < DBL_EPSILON) return (k_unfrozen + k_frozen) / 2.0;
-    return pow(k_unfrozen, theta_l / tw) * pow(k_frozen, theta_i / tw);
+    return POW(k_unfrozen, theta_l / tw) * POW(k_frozen, theta_i / tw);
 }
 
 /* :267-270 */
@@ -199,28 +216,28 @@ double orc_relative_saturation(double theta_l, double theta_i, double nu) {
 double orc_kersten_number(double theta_i, double S_r, double a, double b,
                           double nu_om, double nu_quartz, double nu_gravel) {
     if (theta_i < DBL_EPSILON) {
-        return pow(S_r, (1.0 + nu_om - a * nu_quartz - nu_gravel) / 2.0) *
-               pow(pow(1.0 + exp(-b * S_r), -3.0) - pow((1.0 - S_r) / 2.0, 3.0),
+        return POW(S_r, (1.0 + nu_om - a * nu_quartz - nu_gravel) / 2.0) *
+               POW(POW(1.0 + EXP(-b * S_r), -3.0) - POW((1.0 - S_r) / 2.0, 3.0),
                    1.0 - nu_om);
     }
-    return pow(S_r, 1.0 + nu_om);
+    return POW(S_r, 1.0 + nu_om);
 }
 
 /* :338-348 */
 double orc_kappa_solid(double nu_om, double nu_quartz, double k_om, double k_quartz,
                        double k_minerals) {
-    return pow(k_om, nu_om) * pow(k_quartz, nu_quartz) *
-           pow(k_minerals, 1.0 - nu_om - nu_quartz);
+    return POW(k_om, nu_om) * POW(k_quartz, nu_quartz) *
+           POW(k_minerals, 1.0 - nu_om - nu_quartz);
 }
 
 /* :359-361 */
 double orc_kappa_sat_frozen(double k_solid, double nu, double k_ice) {
-    return pow(k_solid, 1.0 - nu) * pow(k_ice, nu);
+    return POW(k_solid, 1.0 - nu) * POW(k_ice, nu);
 }
 
 /* :371-373 */
 double orc_kappa_sat_unfrozen(double k_solid, double nu, double k_liq) {
-    return pow(k_solid, 1.0 - nu) * pow(k_liq, nu);
+    return POW(k_solid, 1.0 - nu) * POW(k_liq, nu);
 }
 
 /* :386-392 */
@@ -241,7 +258,7 @@ double orc_soil_Tf_depressed(double theta_l, double theta_i, double nu, double t
                              double alpha, double n) {
     double ttot = fmin(RHO_I / RHO_L * theta_i + theta_l, nu);
     double psiw0 = orc_matric_potential(alpha, n, orc_effective_saturation(nu, ttot, theta_r));
-    return fmax(T_FREEZE * exp(GRAV * psiw0 / LH_F0), 1.0);
+    return fmax(T_FREEZE * EXP(GRAV * psiw0 / LH_F0), 1.0);
 }
 
 /* :60-62 */
@@ -254,7 +271,7 @@ double orc_phase_change_source(double theta_l, double theta_i, double T, double 
                                double nu, double theta_r, double alpha, double n,
                                int* err) {
     double Tf = orc_soil_Tf_depressed(theta_l, theta_i, nu, theta_r, alpha, n);
-    double psiT = LH_F0 / GRAV * log(T / Tf) * orc_heaviside(Tf - T);
+    double psiT = LH_F0 / GRAV * LOG(T / Tf) * orc_heaviside(Tf - T);
     double ttot = fmin(RHO_I / RHO_L * theta_i + theta_l, nu);
     double psiw0 = orc_matric_potential(alpha, n, orc_effective_saturation(nu, ttot, theta_r));
     double thstar = orc_inverse_matric_potential(alpha, n, psiw0 + psiT, err) *

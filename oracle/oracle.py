@@ -14,6 +14,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liborc.so")
+LIB_XPREC = os.path.join(HERE, "liborc_xprec.so")  # closures in long double
+LIB_FMA = os.path.join(HERE, "liborc_fma.so")      # a*b+c contracted to FMA
+VARIANTS = {"": LIB, "xprec": LIB_XPREC, "fma": LIB_FMA}
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
@@ -41,7 +44,7 @@ class Forcing(C.Structure):
     _fields_ = [("top_water", _dp), ("top_heat", _dp), ("bot_water", _dp), ("bot_heat", _dp)]
 
 
-_lib = None
+_libs = {}
 
 
 def build():
@@ -50,12 +53,14 @@ def build():
         raise RuntimeError(r.stderr)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def lib(variant: str = ""):
+    """The oracle ("") or one of its rounding probes: "xprec" (closures in
+    long double) or "fma" (fused multiply-add contraction)."""
+    path = VARIANTS[variant]
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         d = C.c_double
         for name, args in {
             "orc_effective_saturation": [d, d, d],
@@ -103,8 +108,8 @@ def lib():
                               C.c_int64, C.c_int]
         L.orc_run.restype = C.c_int64
         L.orc_totals.argtypes = [M, S, _dp, _dp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def constants() -> dict:
@@ -123,7 +128,8 @@ def _p(a):
 class OracleModel:
     """Oracle view of a CSR column batch (same arrays the library takes)."""
 
-    def __init__(self, model: int, bc_bottom: int, col_ptr, z_face, params):
+    def __init__(self, model: int, bc_bottom: int, col_ptr, z_face, params, variant=""):
+        self.L = lib(variant)
         self.col_ptr = np.ascontiguousarray(col_ptr, dtype=np.int32)
         self.z_face = np.ascontiguousarray(z_face, dtype=np.float64)
         self.params = np.ascontiguousarray(params, dtype=np.float64)
@@ -135,8 +141,8 @@ class OracleModel:
                         _p(self.z_face), _p(self.params))
 
     @classmethod
-    def from_batch(cls, b):
-        return cls(b.model, b.bc_bottom, b.col_ptr, b.z_face, b.params)
+    def from_batch(cls, b, variant=""):
+        return cls(b.model, b.bc_bottom, b.col_ptr, b.z_face, b.params, variant=variant)
 
     def new_state(self, src: dict | None = None) -> dict:
         st = {"vartheta_l": np.zeros(self.ncell), "intF_water": np.zeros(self.ncol)}
@@ -168,16 +174,16 @@ class OracleModel:
 
     def update_cache(self, Y: dict, aux: dict, tw, th=None, bw=None, bh=None):
         f = self._f(tw, th, bw, bh)
-        lib().orc_update_cache(C.byref(self._m), C.byref(self._s(Y)), C.byref(f),
+        self.L.orc_update_cache(C.byref(self._m), C.byref(self._s(Y)), C.byref(f),
                                C.byref(self._a(aux)))
 
     def update_implicit_cache(self, Y: dict, aux: dict):
-        lib().orc_update_implicit_cache(C.byref(self._m), C.byref(self._s(Y)),
+        self.L.orc_update_implicit_cache(C.byref(self._m), C.byref(self._s(Y)),
                                         C.byref(self._a(aux)))
 
     def exp_tendency(self, Y: dict, aux: dict) -> dict:
         dY = self.new_state()
-        r = lib().orc_exp_tendency(C.byref(self._m), C.byref(self._s(Y)), C.byref(self._a(aux)),
+        r = self.L.orc_exp_tendency(C.byref(self._m), C.byref(self._s(Y)), C.byref(self._a(aux)),
                                    C.byref(self._s(dY)))
         if r:
             raise ValueError("Matric potential is positive")
@@ -185,7 +191,7 @@ class OracleModel:
 
     def imp_tendency(self, Y: dict, aux: dict) -> dict:
         dY = self.new_state()
-        lib().orc_imp_tendency(C.byref(self._m), C.byref(self._s(Y)), C.byref(self._a(aux)),
+        self.L.orc_imp_tendency(C.byref(self._m), C.byref(self._s(Y)), C.byref(self._a(aux)),
                                C.byref(self._s(dY)))
         return dY
 
@@ -193,14 +199,14 @@ class OracleModel:
         ww = np.zeros(3 * self.ncell)
         ee = np.zeros(3 * self.ncell) if self.eh else None
         ew = np.zeros(3 * self.ncell) if self.eh else None
-        lib().orc_jacobian(C.byref(self._m), C.byref(self._s(Y)), C.byref(self._a(aux)), dtg,
+        self.L.orc_jacobian(C.byref(self._m), C.byref(self._s(Y)), C.byref(self._a(aux)), dtg,
                            _p(ww), _p(ee), _p(ew))
         return ww, ee, ew
 
     def solve(self, W, R: dict) -> dict:
         ww, ee, ew = W
         dx = self.new_state()
-        lib().orc_solve(C.byref(self._m), _p(ww), _p(ee), _p(ew), C.byref(self._s(R)),
+        self.L.orc_solve(C.byref(self._m), _p(ww), _p(ee), _p(ew), C.byref(self._s(R)),
                         C.byref(self._s(dx)))
         return dx
 
@@ -208,7 +214,7 @@ class OracleModel:
              bh=None):
         it = np.zeros(self.ncol, dtype=np.int32)
         f = self._f(tw, th, bw, bh)
-        nc = lib().orc_step(C.byref(self._m), C.byref(self._s(Y)), C.byref(f), dt, max_iters,
+        nc = self.L.orc_step(C.byref(self._m), C.byref(self._s(Y)), C.byref(f), dt, max_iters,
                             tol, it.ctypes.data_as(_ip))
         if nc < 0:
             raise ValueError("Matric potential is positive")
@@ -220,7 +226,7 @@ class OracleModel:
         tw = np.ascontiguousarray(top_water, dtype=np.float64)
         th = None if top_heat is None else np.ascontiguousarray(top_heat, dtype=np.float64)
         it = np.zeros((nsteps, self.ncol), dtype=np.int32) if want_iters else None
-        nc = lib().orc_run(C.byref(self._m), C.byref(self._s(Y)), _p(tw), _p(th), tw.shape[0],
+        nc = self.L.orc_run(C.byref(self._m), C.byref(self._s(Y)), _p(tw), _p(th), tw.shape[0],
                            steps_per_slot, dt, nsteps, max_iters, tol,
                            it.ctypes.data_as(_ip) if it is not None else None, ncol_limit,
                            nthreads)
@@ -231,5 +237,5 @@ class OracleModel:
     def totals(self, Y: dict):
         w = np.zeros(self.ncol)
         e = np.zeros(self.ncol) if self.eh else None
-        lib().orc_totals(C.byref(self._m), C.byref(self._s(Y)), _p(w), _p(e))
+        self.L.orc_totals(C.byref(self._m), C.byref(self._s(Y)), _p(w), _p(e))
         return w, e

@@ -49,7 +49,7 @@ def test_header_enums_match_binding():
 
 def test_struct_layouts_match_header():
     from clmsoil import _lib
-    assert C.sizeof(_lib.Config) == 4 + 4 + 8 + 8 * 3 + 4 * 3 + 4 + 8  # incl. padding
+    assert C.sizeof(_lib.Config) == 4 + 4 + 8 + 8 * 3 + 4 * 3 + 4 + 8 + 4 + 4  # incl. padding
     assert C.sizeof(_lib.State) == 5 * 8
     assert C.sizeof(_lib.Aux) == 9 * 8
     assert C.sizeof(_lib.StepStats) == 4 + 4 + 8 + 8 * 3 + 8
@@ -61,6 +61,7 @@ def test_struct_layouts_match_header():
     (lambda c: setattr(c, "nparam", 3), -1),
     (lambda c: setattr(c, "jac_precision", 16), -1),
     (lambda c: setattr(c, "newton_max_iters", 0), -1),
+    (lambda c: setattr(c, "solver", 9), -1),
 ])
 def test_create_rejects_bad_config_without_device(mutate, code):
     """Validation runs before any HIP call, so this is safe without a GPU."""
@@ -69,7 +70,7 @@ def test_create_rejects_bad_config_without_device(mutate, code):
     cp = np.array([0, 3, 7], dtype=np.int32)
     zf = np.array([-1, -0.5, -0.2, 0, -2, -1.5, -1, -0.5, 0.0])
     P = np.ones((6, 2))
-    cfg = _lib.Config(0, 1, 2, _lib.iptr(cp), _lib.dptr(zf), _lib.dptr(P), 6, 64, 3, 0.0)
+    cfg = _lib.Config(0, 1, 2, _lib.iptr(cp), _lib.dptr(zf), _lib.dptr(P), 6, 64, 3, 0.0, 0)
     mutate(cfg)
     ctx = C.c_void_p()
     assert lib.clm_create(C.byref(ctx), C.byref(cfg), None) == code
@@ -83,14 +84,14 @@ def test_create_rejects_bad_geometry_without_device():
     cp = np.array([0, 3, 4], dtype=np.int32)   # second column has 1 layer
     zf = np.linspace(-1, 0, 6)
     P = np.ones((6, 2))
-    cfg = _lib.Config(0, 1, 2, _lib.iptr(cp), _lib.dptr(zf), _lib.dptr(P), 6, 64, 3, 0.0)
+    cfg = _lib.Config(0, 1, 2, _lib.iptr(cp), _lib.dptr(zf), _lib.dptr(P), 6, 64, 3, 0.0, 0)
     ctx = C.c_void_p()
     assert lib.clm_create(C.byref(ctx), C.byref(cfg), None) == -1
     assert b"layers" in lib.clm_last_error(None)
     cp = np.array([0, 3], dtype=np.int32)
     zf = np.array([0.0, -0.5, -0.7, -1.0])      # decreasing faces
     cfg = _lib.Config(0, 1, 1, _lib.iptr(cp), _lib.dptr(zf), _lib.dptr(np.ones((6, 1))), 6, 64,
-                      3, 0.0)
+                      3, 0.0, 0)
     assert lib.clm_create(C.byref(ctx), C.byref(cfg), None) == -1
     assert b"increasing" in lib.clm_last_error(None)
 

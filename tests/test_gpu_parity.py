@@ -17,11 +17,14 @@ from conftest import column_ok, rel_err
 pytestmark = pytest.mark.gpu
 
 
-def _models(batch, jac=64, max_iters=None, tol=None):
+def _models(batch, jac=64, max_iters=None, tol=None, solver=0):
     from clmsoil.model import model_from_batch
     from oracle.oracle import OracleModel
-    return model_from_batch(batch, jac_precision=jac, max_iters=max_iters, tol=tol), \
-        OracleModel.from_batch(batch)
+    return model_from_batch(batch, jac_precision=jac, max_iters=max_iters, tol=tol,
+                            solver=solver), OracleModel.from_batch(batch)
+
+
+SOLVERS = [pytest.param(1, id="group"), pytest.param(2, id="lane")]
 
 
 def _cell_mask(batch, colmask):
@@ -151,26 +154,33 @@ def _col_max(b, cellvals, colmask):
     return np.maximum.reduceat(v, b.col_ptr[:-1])
 
 
-def _per_step_parity(name, ncol, jac, max_iters, tol, nsteps, keys, label):
+def _per_step_parity(name, ncol, jac, max_iters, tol, nsteps, keys, label, solver=0,
+                     vmax=1e300, min_well=0.9):
     """Per-step map parity: every step, the GPU and the oracle advance the
     SAME input state (the oracle's state after the previous step).
 
     Column-steps that stay finite, converge (tol mode: same Newton count on
     both sides, below max_iters) and are well-conditioned -- the oracle's
-    own output moves by <= 1e-12 when its input is perturbed by 1 ulp --
+    own output moves by <= 1e-12 when its input is perturbed by 1 ulp, when
+    its closures are evaluated in long double, and when it is compiled with
+    fused multiply-add contraction (as GPU code is) --
     must agree to 1e-10 relative on >= 99.9% of them and to 1e-6 on all of
     them (the rare exceptions are iterates that straddle a branch point of
     the closures, e.g. S = 1 where dψ/dϑ jumps, which a single 1-ulp probe
     need not flip).  Column-steps that amplify rounding more are held to
     1e3 x that self-sensitivity.  Newton-count mismatches must not exceed
-    twice the oracle's own mismatch under the 1-ulp probe plus 0.5%.  Ice
+    twice the oracle's own mismatch under those probes plus 0.5%.  Ice
     fractions below 1e-3 are compared in absolute terms (floor 1e-3),
-    ρe_int with a 1 J/m3 floor."""
+    ρe_int with a 1 J/m3 floor.  `vmax` drops columns whose liquid water
+    has left [-vmax, vmax] (iterates that diverged)."""
     from clmsoil import inputs
     from clmsoil.model import LandSimulation
     b = inputs.make_batch(name, ncol=ncol, nsteps=nsteps)
     w = b.workload
-    m, o = _models(b, jac=jac, max_iters=max_iters, tol=tol)
+    m, o = _models(b, jac=jac, max_iters=max_iters, tol=tol, solver=solver)
+    from oracle.oracle import OracleModel
+    ox = OracleModel.from_batch(b, variant="xprec")
+    of = OracleModel.from_batch(b, variant="fma")
     sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=max_iters, tol=tol)
     Yo = b.copy_state()
     floors = {"vartheta_l": 1e-12, "theta_i": 1e-3, "rho_e_int": 1.0, "intF_water": 1e-12,
@@ -190,15 +200,20 @@ def _per_step_parity(name, ncol, jac, max_iters, tol, nsteps, keys, label):
         Yp["vartheta_l"] *= 1.0 + 2.220446049250313e-16 * flip
         if "rho_e_int" in Yp:
             Yp["rho_e_int"] *= 1.0 + 2.220446049250313e-16 * flip
-        ok = column_ok(b, Yo, vmax=1e300)
+        Yx = {k: v.copy() for k, v in Yo.items()}
+        Yf = {k: v.copy() for k, v in Yo.items()}
+        ok = column_ok(b, Yo, vmax=vmax)
         fw, fh = b.top_water[sl].copy(), None if th is None else th[0].copy()
         it_o, _ = o.step(Yo, w.dt, max_iters, tol, fw, fh)
         it_p, _ = o.step(Yp, w.dt, max_iters, tol, fw, fh)
-        ok &= column_ok(b, Yo, vmax=1e300) & column_ok(b, Yg, vmax=1e300)
-        ok &= column_ok(b, Yp, vmax=1e300)
+        it_x, _ = ox.step(Yx, w.dt, max_iters, tol, fw, fh)
+        it_f, _ = of.step(Yf, w.dt, max_iters, tol, fw, fh)
+        ok &= column_ok(b, Yo, vmax=vmax) & column_ok(b, Yg, vmax=vmax)
+        ok &= column_ok(b, Yp, vmax=vmax) & column_ok(b, Yx, vmax=vmax)
+        ok &= column_ok(b, Yf, vmax=vmax)
         same = (it_g & 0xFFFF) == it_o
         nsame += (same & ok).sum()
-        nself += ((it_p == it_o) & ok).sum()
+        nself += ((it_p == it_o) & (it_x == it_o) & (it_f == it_o) & ok).sum()
         ntot += ok.sum()
         conv = ok & same
         if tol > 0:
@@ -209,7 +224,9 @@ def _per_step_parity(name, ncol, jac, max_iters, tol, nsteps, keys, label):
         for k in keys:
             cell = Yo[k].size == b.ncell
             e = rel_err(Yg[k], Yo[k], floors[k])
-            d = rel_err(Yp[k], Yo[k], floors[k])
+            d = np.maximum(np.maximum(rel_err(Yp[k], Yo[k], floors[k]),
+                                      rel_err(Yx[k], Yo[k], floors[k])),
+                           rel_err(Yf[k], Yo[k], floors[k]))
             if cell:
                 e = _col_max(b, e, conv)
                 d = _col_max(b, d, conv)
@@ -234,36 +251,61 @@ def _per_step_parity(name, ncol, jac, max_iters, tol, nsteps, keys, label):
     print(f"{label}: per-step worst rel err {{{', '.join(f'{k}: {v:.2e}' for k, v in worst.items())}}} "
           f"on {nwell - nout} of {nwell} well-conditioned column-steps ({nout} above 1e-10, all "
           f"<= 1e-6); {nconv} converged of {ntot} finite; equal Newton counts "
-          f"{nsame / max(ntot, 1):.4f} (oracle vs its 1-ulp probe {nself / max(ntot, 1):.4f}); "
+          f"{nsame / max(ntot, 1):.4f} (oracle vs its own probes {nself / max(ntot, 1):.4f}); "
           f"worst err/self-sensitivity elsewhere {worst_ill:.2g}", flush=True)
     assert nout <= 0.001 * nwell
     assert (ntot - nsame) <= 2 * (ntot - nself) + 0.005 * ntot
-    assert nwell >= 0.9 * nconv
+    assert nwell >= min_well * nconv
 
 
-def test_rre_per_step_fixed_iters(gpu_lib):
-    """rre_4096x50 semantics: 3 fixed Newton iterations, all 96 steps."""
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_rre_per_step_fixed_iters(gpu_lib, solver):
+    """rre_4096x50 semantics: 3 fixed Newton iterations, all 96 steps.
+    Without a convergence check some columns' iterates diverge (|ϑ| reaches
+    1e21, in the oracle as well); column-steps with |ϑ| > 2 are left out."""
     _per_step_parity("rre_4096x50", 256, 64, 3, 0.0, 96, ("vartheta_l", "intF_water"),
-                     "rre fixed")
+                     f"rre fixed s{solver}", solver, vmax=2.0)
 
 
-def test_rre_per_step_tol(gpu_lib):
-    """rre_131072_uneven semantics: Newton to 1e-8, 48 steps."""
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_rre_per_step_tol(gpu_lib, solver):
+    """rre_131072_uneven semantics: Newton to 1e-8, 48 steps (layers 15-120:
+    every group shape incl. two rows per lane)."""
     _per_step_parity("rre_131072_uneven", 256, 64, 50, 1e-8, 48,
-                     ("vartheta_l", "intF_water"), "rre tol")
+                     ("vartheta_l", "intF_water"), f"rre tol s{solver}", solver)
 
 
-def test_eh_per_step_fp64(gpu_lib):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_eh_per_step_fp64(gpu_lib, solver):
     """eh_65536x20 semantics with an FP64 Jacobian, 96 steps."""
     _per_step_parity("eh_65536x20", 192, 64, 50, 1e-8, 96,
                      ("vartheta_l", "theta_i", "rho_e_int", "intF_water", "intF_energy"),
-                     "eh fp64")
+                     f"eh fp64 s{solver}", solver)
 
 
-def test_eh_per_step_fp64_uneven(gpu_lib):
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_eh_per_step_fp64_uneven(gpu_lib, solver):
     """eh_262144_uneven semantics with an FP64 Jacobian, first 60 steps."""
     _per_step_parity("eh_262144_uneven", 192, 64, 50, 1e-8, 60,
-                     ("vartheta_l", "theta_i", "rho_e_int"), "eh uneven fp64")
+                     ("vartheta_l", "theta_i", "rho_e_int"), f"eh uneven fp64 s{solver}", solver)
+
+
+def test_eh_per_step_deep_two_rows_per_lane(gpu_lib):
+    """EnergyHydrology on columns of 65-128 layers (group solver, two rows
+    per lane): the rre_131072_uneven geometry under the EH physics."""
+    import dataclasses
+    from clmsoil import inputs
+    w = inputs.WORKLOADS["eh_65536x20"]
+    inputs.WORKLOADS["eh_deep_test"] = dataclasses.replace(
+        w, name="eh_deep_test", config_id=7, layers=(65, 128), depth=(5.0, 10.0), ncol=128)
+    try:
+        # deep columns under freeze-thaw are often ill-conditioned: the bar
+        # applies to the well-conditioned column-steps, which must be >= 75%
+        _per_step_parity("eh_deep_test", 96, 64, 50, 1e-8, 30,
+                         ("vartheta_l", "theta_i", "rho_e_int"), "eh deep group", 1,
+                         min_well=0.75)
+    finally:
+        del inputs.WORKLOADS["eh_deep_test"]
 
 
 def _run_both(name, ncol, jac=64, nsteps=None):
