@@ -61,6 +61,15 @@ def run_form(batch, jac, steps, warmup, solver=0):
     sim = LandSimulation(m, batch.copy_state(), w.dt, max_iters=w.newton_max_iters,
                          tol=w.newton_tol)
     sim.set_forcing_table(batch.top_water, batch.top_heat)
+    shapes = None
+    if m.shapes():  # group solver: autotune the per-depth shapes on the initial state
+        m.boundary_conditions["top"] = _top_bc(batch)
+        m.set_forcing()
+        sim._push()
+        m.autotune(w.dt, reps=2)
+        shapes = [s for s in m.shapes() if s[3] > 0]
+        print(f"[bench] autotuned shapes (max layers, lanes/column, rows/lane, columns): "
+              f"{shapes}", file=sys.stderr, flush=True)
     if warmup > 0:  # untimed warmup steps
         sim.solve(None, None, warmup, w.steps_per_slot, upload=False)
     # capture the K-step graph outside the timed region, then restart from u0
@@ -78,7 +87,14 @@ def run_form(batch, jac, steps, warmup, solver=0):
     _, iters = sim.solve(None, None, steps, w.steps_per_slot, want_iters=True, upload=False)
     kernel_ms = st.ms / steps
     m.close()
-    return dict(wall=wall, stats=st, iters=iters, kernel_ms=kernel_ms)
+    return dict(wall=wall, stats=st, iters=iters, kernel_ms=kernel_ms, shapes=shapes)
+
+
+def _top_bc(batch):
+    from clmsoil.model import HeatFluxBC, WaterFluxBC, WaterHeatBC
+    if batch.top_heat is None:
+        return WaterFluxBC(batch.top_water[0])
+    return WaterHeatBC(WaterFluxBC(batch.top_water[0]), HeatFluxBC(batch.top_heat[0]))
 
 
 def main():
@@ -140,7 +156,9 @@ def main():
                    "dt_s": w.dt, "newton_tol": w.newton_tol,
                    "newton_max_iters": w.newton_max_iters,
                    "form": args.form + (" (no learned warm start yet)" if args.form == "mixed"
-                                        else "")},
+                                        else ""),
+                   "solver": "group" if r.get("shapes") else "lane",
+                   "shapes": r.get("shapes")},
         "plain_fp64_value": batch.ncol * steps / res["fp64"]["wall"],
         "speedup_vs_plain_fp64": value / (batch.ncol * steps / res["fp64"]["wall"]),
         "newton": {"mean_iters": st.mean_iters, "max_iters": st.max_iters_used,

@@ -27,7 +27,7 @@ EXPORTED = (
     "clm_get_state", "clm_set_forcing", "clm_set_forcing_table", "clm_update_cache",
     "clm_exp_tendency", "clm_imp_tendency", "clm_jacobian", "clm_solve",
     "clm_update_implicit_cache", "clm_get_aux", "clm_get_jacobian", "clm_step", "clm_run",
-    "clm_totals", "clm_last_kernel_ms",
+    "clm_totals", "clm_last_kernel_ms", "clm_get_shapes", "clm_set_shape", "clm_autotune",
 )
 
 _dp = C.POINTER(C.c_double)
@@ -103,6 +103,9 @@ def load(path: str | None = None) -> C.CDLL:
                      _ip], C.c_int),
         "clm_totals": ([vp, _dp, _dp], C.c_int),
         "clm_last_kernel_ms": ([vp, _dp, _ip], C.c_int),
+        "clm_get_shapes": ([vp, _ip, C.c_int32, _ip], C.c_int),
+        "clm_set_shape": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
+        "clm_autotune": ([vp, C.c_double, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

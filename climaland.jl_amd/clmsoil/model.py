@@ -223,6 +223,23 @@ class _SoilModel:
         self._check(self._lib.clm_get_aux(self._ctx, C.byref(s)))
         return a
 
+    # -- group-solver shapes and autotuner
+    def shapes(self):
+        """[(bucket max layers, G lanes/column, R rows/lane, columns)]."""
+        n = C.c_int32(0)
+        self._check(self._lib.clm_get_shapes(self._ctx, None, 0, C.byref(n)))
+        rec = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+        self._check(self._lib.clm_get_shapes(self._ctx, L.iptr(rec), n.value, C.byref(n)))
+        return [tuple(int(v) for v in rec[4 * i:4 * i + 4]) for i in range(n.value)]
+
+    def set_shape(self, bucket_max_layers: int, G: int, R: int):
+        self._check(self._lib.clm_set_shape(self._ctx, bucket_max_layers, G, R))
+
+    def autotune(self, dt: float, reps: int = 2):
+        """Time every fitting shape per depth bucket on one step from the
+        current device state (pushed by LandSimulation) and keep the fastest."""
+        self._check(self._lib.clm_autotune(self._ctx, dt, reps))
+
     def get_jacobian(self):
         n = 3 * self.domain.ncell
         ww = np.zeros(n)

@@ -86,13 +86,54 @@ __device__ __forceinline__ void pcr2(int gl, const JT* a, const JT* b, const JT*
     x[0] = (d[0] - a[0] * xm - c[0] * xo) / b[0];
 }
 
+// R >= 2 rows per lane: the partition method.  The lane's interior rows
+// 1..R-2 form a tridiagonal system with its first and last unknowns as
+// boundary values; three Thomas sweeps (right-hand sides d, -a_1 e_1,
+// -c_{R-2} e_{R-2}) give x_k = α_k + β_k x_0 + γ_k x_{R-1}.  Substituting
+// into rows 0 and R-1 leaves two rows per lane, tridiagonal in the order
+// (.., x_{R-1} of lane l-1, x_0, x_{R-1}, x_0 of lane l+1, ..): the Schur
+// complement of a diagonally dominant matrix, solved by pcr2.  Interior
+// unknowns follow from the formula above.
 template <int G, int R, typename JT>
 __device__ __forceinline__ void pcr_solve(int gl, JT (&a)[R], JT (&b)[R], JT (&c)[R], JT (&d)[R],
                                           JT (&x)[R]) {
     if constexpr (R == 1) {
         x[0] = pcr1<G, JT>(gl, a[0], b[0], c[0], d[0]);
-    } else {
+    } else if constexpr (R == 2) {
         pcr2<G, JT>(gl, a, b, c, d, x);
+    } else {
+        constexpr int M = R - 2;  // interior rows 1..R-2
+        JT cp[M], al[M], be[M], ga[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const int k = j + 1;
+            JT den = b[k] - (j > 0 ? a[k] * cp[j - 1] : (JT)0);
+            JT inv = (JT)1 / den;
+            cp[j] = c[k] * inv;
+            al[j] = (d[k] - (j > 0 ? a[k] * al[j - 1] : (JT)0)) * inv;
+            be[j] = ((j == 0 ? -a[1] : (JT)0) - (j > 0 ? a[k] * be[j - 1] : (JT)0)) * inv;
+            ga[j] = ((j == M - 1 ? -c[R - 2] : (JT)0) - (j > 0 ? a[k] * ga[j - 1] : (JT)0)) * inv;
+        }
+#pragma unroll
+        for (int j = M - 2; j >= 0; --j) {
+            al[j] -= cp[j] * al[j + 1];
+            be[j] -= cp[j] * be[j + 1];
+            ga[j] -= cp[j] * ga[j + 1];
+        }
+        JT ra[2], rb[2], rc[2], rd[2], rx[2];
+        ra[0] = a[0];
+        rb[0] = b[0] + c[0] * be[0];
+        rc[0] = c[0] * ga[0];
+        rd[0] = d[0] - c[0] * al[0];
+        ra[1] = a[R - 1] * be[M - 1];
+        rb[1] = b[R - 1] + a[R - 1] * ga[M - 1];
+        rc[1] = c[R - 1];
+        rd[1] = d[R - 1] - a[R - 1] * al[M - 1];
+        pcr2<G, JT>(gl, ra, rb, rc, rd, rx);
+        x[0] = rx[0];
+        x[R - 1] = rx[1];
+#pragma unroll
+        for (int j = 0; j < M; ++j) x[j + 1] = al[j] + be[j] * rx[0] + ga[j] * rx[1];
     }
 }
 
@@ -256,7 +297,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     const double topw = ftw[col], toph = fth[col];
 
     double xv[R], xi[R], xe[R], u0v[R], u0e[R], zc[R], dzc[R], dzf[R];
-    double Kl[R], kl[R], rcs[R], Tp[R], Pp[R];
+    double Kl[R], kl[R], rcs[R], Tp[R], Pp[R], D[R];
     bool on[R];
     int bad = 0;
     // ---- stage 1: update_cache!(u_n) + phase change -> U0 -------------------
@@ -290,6 +331,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         rcs[k] = heat_capacity(a.thl, xi[k], e.rhoc_ds);  // 1/D of the ρe block (lagged θ_l)
         Tp[k] = a.T;
         Pp[k] = a.psi;
+        D[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).D;  // Wfact at x = U0
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);
@@ -318,10 +360,9 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     bool conv = false;
     double err = 0.0;
     for (kit = 1; kit <= max_iters; ++kit) {
-        double D[R], h[R], E[R];
+        double h[R], E[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            D[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).D;  // Wfact(x): dψ/dϑ at the iterate
             h[k] = Pp[k] + zc[k];                          // p.ψ
             E[k] = energy_liq(Tp[k]) * Kl[k];              // p.T
         }
@@ -407,9 +448,12 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             double yr = on[k] ? (double)y[k] : 0.0;
             xe[k] -= yr * rcs[k];
             e2 = nan_max(e2, fabs(yr));
-            // cache_imp!(x): T and ψ at the new iterate (energy_hydrology.jl:427-445)
+            // cache_imp!(x): T and ψ at the new iterate (energy_hydrology.jl:427-445);
+            // dψ/dϑ of the same evaluation is the next Wfact(x)
             Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
-            Pp[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).psi;
+            PsiD pd = psi_dpsi(e.v, e.nu - xi[k], xv[k]);
+            Pp[k] = pd.psi;
+            D[k] = pd.D;
         }
         err = g_max<G>(e2);
         conv = (tol > 0.0) && (err < tol);

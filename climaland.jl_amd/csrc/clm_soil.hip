@@ -81,7 +81,8 @@ struct clm_ctx {
         int32_t n;
         int32_t* d_cols;
     };
-    std::vector<Bin> bins;
+    std::vector<Bin> bins;  // one per depth bucket (n may be 0)
+    int bucket_G[16], bucket_R[16], bucket_nmax[16];
     // stream, graph, events
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -145,13 +146,30 @@ bool is_group(const clm_ctx* c) { return c->solver == CLM_SOLVER_GROUP; }
 // length of per-column device vectors of the coarse path
 int64_t ncolvec(const clm_ctx* c) { return is_group(c) ? c->ncol : c->nslot; }
 
-// group-solver bins: (lanes per column G, rows per lane R) for a column of N layers
-void bin_shape(int N, int* G, int* R) {
-    if (N <= 8) { *G = 8; *R = 1; }
-    else if (N <= 16) { *G = 16; *R = 1; }
-    else if (N <= 32) { *G = 32; *R = 1; }
-    else if (N <= 64) { *G = 64; *R = 1; }
-    else { *G = 64; *R = 2; }
+// Group-solver shapes: G lanes per column, R rows per lane (capacity G*R).
+struct Shape {
+    int G, R;
+};
+constexpr Shape kMenu[] = {{8, 1},  {16, 1}, {32, 1}, {64, 1}, {8, 2},  {16, 2}, {32, 2},
+                           {64, 2}, {8, 3},  {16, 3}, {32, 3}, {4, 4},  {8, 4},  {16, 4},
+                           {32, 4}, {8, 5},  {16, 5}, {8, 6},  {8, 8},  {16, 8}};
+constexpr int kNMenu = sizeof(kMenu) / sizeof(kMenu[0]);
+// depth buckets: columns with kBucketMax[b-1] < N <= kBucketMax[b] share a shape
+constexpr int kBucketMax[] = {8, 16, 24, 32, 40, 48, 64, 80, 96, 128};
+constexpr int kNBucket = sizeof(kBucketMax) / sizeof(kBucketMax[0]);
+// shapes before autotuning: few lanes per column, several rows per lane
+constexpr Shape kDefaultShape[kNBucket] = {{8, 1},  {8, 2},  {8, 3},  {8, 4},  {16, 3},
+                                           {16, 3}, {16, 4}, {32, 3}, {32, 3}, {32, 4}};
+
+int bucket_of(int N) {
+    for (int b = 0; b < kNBucket; ++b)
+        if (N <= kBucketMax[b]) return b;
+    return -1;
+}
+int menu_index(int G, int R) {
+    for (int i = 0; i < kNMenu; ++i)
+        if (kMenu[i].G == G && kMenu[i].R == R) return i;
+    return -1;
 }
 size_t jt_size(const clm_ctx* c) { return c->jac_prec == 32 ? sizeof(float) : sizeof(double); }
 
@@ -391,17 +409,27 @@ void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const dou
     }
 }
 
+template <typename JT, int I>
+void launch_menu(clm_ctx* c, int idx, const clm_ctx::Bin& bn, const double* ftw,
+                 const double* fth, double dt, int32_t* iters) {
+    if constexpr (I < kNMenu) {
+        if (idx == I)
+            launch_bin<JT, kMenu[I].G, kMenu[I].R>(c, bn, ftw, fth, dt, iters);
+        else
+            launch_menu<JT, I + 1>(c, idx, bn, ftw, fth, dt, iters);
+    }
+}
+
+template <typename JT>
+void launch_bin_any(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const double* fth,
+                    double dt, int32_t* iters) {
+    if (bn.n > 0) launch_menu<JT, 0>(c, menu_index(bn.G, bn.R), bn, ftw, fth, dt, iters);
+}
+
 template <typename JT>
 void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
                     int32_t* iters) {
-    for (const auto& bn : c->bins) {
-        if (bn.n == 0) continue;
-        if (bn.G == 8) launch_bin<JT, 8, 1>(c, bn, ftw, fth, dt, iters);
-        else if (bn.G == 16) launch_bin<JT, 16, 1>(c, bn, ftw, fth, dt, iters);
-        else if (bn.G == 32) launch_bin<JT, 32, 1>(c, bn, ftw, fth, dt, iters);
-        else if (bn.R == 1) launch_bin<JT, 64, 1>(c, bn, ftw, fth, dt, iters);
-        else launch_bin<JT, 64, 2>(c, bn, ftw, fth, dt, iters);
-    }
+    for (const auto& bn : c->bins) launch_bin_any<JT>(c, bn, ftw, fth, dt, iters);
 }
 
 void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters) {
@@ -611,20 +639,22 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
             ALLOC(c->d_ce, c->ncell);
             ALLOC(c->d_cie, c->ncol);
         }
-        // depth bins
-        const int shapes[5][2] = {{8, 1}, {16, 1}, {32, 1}, {64, 1}, {64, 2}};
-        std::vector<std::vector<int32_t>> cols(5);
+        // depth buckets and their column lists
+        std::vector<std::vector<int32_t>> cols(kNBucket);
+        for (int b = 0; b < kNBucket; ++b) c->bucket_nmax[b] = 0;
         for (int64_t i = 0; i < c->ncol; ++i) {
-            int G, R;
-            bin_shape(c->col_ptr[i + 1] - c->col_ptr[i], &G, &R);
-            for (int k = 0; k < 5; ++k)
-                if (shapes[k][0] == G && shapes[k][1] == R) cols[k].push_back((int32_t)i);
+            int Ni = c->col_ptr[i + 1] - c->col_ptr[i];
+            int b = bucket_of(Ni);
+            cols[b].push_back((int32_t)i);
+            c->bucket_nmax[b] = std::max(c->bucket_nmax[b], Ni);
         }
-        for (int k = 0; k < 5; ++k) {
-            clm_ctx::Bin bn{shapes[k][0], shapes[k][1], (int32_t)cols[k].size(), nullptr};
+        for (int b = 0; b < kNBucket; ++b) {
+            c->bucket_G[b] = kDefaultShape[b].G;
+            c->bucket_R[b] = kDefaultShape[b].R;
+            clm_ctx::Bin bn{c->bucket_G[b], c->bucket_R[b], (int32_t)cols[b].size(), nullptr};
             if (bn.n > 0) {
                 ALLOC(bn.d_cols, bn.n);
-                if (hipMemcpyAsync(bn.d_cols, cols[k].data(), 4 * cols[k].size(),
+                if (hipMemcpyAsync(bn.d_cols, cols[b].data(), 4 * cols[b].size(),
                                    hipMemcpyHostToDevice, c->stream) != hipSuccess ||
                     hipStreamSynchronize(c->stream) != hipSuccess) {
                     set_err(c, CLM_E_HIP, "bin upload failed");
@@ -1211,6 +1241,116 @@ int clm_totals(clm_ctx* c, double* water, double* energy) {
         water[col] = w;
         if (energy) energy[col] = en;
     }
+    return CLM_OK;
+}
+
+int clm_get_shapes(clm_ctx* c, int32_t* rec, int32_t max_records, int32_t* nrec) {
+    if (!c || !nrec) return CLM_E_ARG;
+    int32_t n = 0;
+    if (is_group(c)) {
+        for (int b = 0; b < kNBucket; ++b) {
+            if (rec && n < max_records) {
+                rec[4 * n + 0] = kBucketMax[b];
+                rec[4 * n + 1] = c->bins[b].G;
+                rec[4 * n + 2] = c->bins[b].R;
+                rec[4 * n + 3] = c->bins[b].n;
+            }
+            ++n;
+        }
+    }
+    *nrec = n;
+    return CLM_OK;
+}
+
+int clm_set_shape(clm_ctx* c, int32_t bucket_max_layers, int32_t G, int32_t R) {
+    if (!c) return CLM_E_ARG;
+    if (!is_group(c)) return set_err(c, CLM_E_STATE, "shapes apply to the group solver only");
+    int b = -1;
+    for (int k = 0; k < kNBucket; ++k)
+        if (kBucketMax[k] == bucket_max_layers) b = k;
+    if (b < 0) return set_err(c, CLM_E_ARG, "no depth bucket with max %d layers", bucket_max_layers);
+    if (menu_index(G, R) < 0) return set_err(c, CLM_E_ARG, "shape (%d, %d) is not compiled", G, R);
+    if (G * R < c->bucket_nmax[b])
+        return set_err(c, CLM_E_ARG, "shape (%d, %d) holds %d rows < %d layers", G, R, G * R,
+                       c->bucket_nmax[b]);
+    c->bins[b].G = G;
+    c->bins[b].R = R;
+    destroy_graph(c);
+    return CLM_OK;
+}
+
+int clm_autotune(clm_ctx* c, double dt, int32_t reps) {
+    if (!c || !(dt > 0.0) || reps < 1) return CLM_E_ARG;
+    if (!is_group(c)) return CLM_OK;
+    int r = ensure_iters(c, 1);
+    if (r) return r;
+    // save the state (every trial starts from it)
+    const int64_t nc = c->ncell, nk = c->ncol;
+    std::vector<std::pair<double*, int64_t>> st = {{c->d_cv, nc}, {c->d_ciw, nk}};
+    if (is_eh(c)) {
+        st.push_back({c->d_ci, nc});
+        st.push_back({c->d_ce, nc});
+        st.push_back({c->d_cie, nk});
+    }
+    int64_t tot = 0;
+    for (auto& p : st) tot += p.second;
+    double* save = nullptr;
+    HIPCHK(c, hipMalloc(&save, 8 * (size_t)tot));
+    auto copy_all = [&](bool to_save) -> hipError_t {
+        int64_t off = 0;
+        for (auto& p : st) {
+            hipError_t e = to_save ? hipMemcpyAsync(save + off, p.first, 8 * p.second,
+                                                    hipMemcpyDeviceToDevice, c->stream)
+                                   : hipMemcpyAsync(p.first, save + off, 8 * p.second,
+                                                    hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) return e;
+            off += p.second;
+        }
+        return hipSuccess;
+    };
+    hipError_t e = copy_all(true);
+    for (int b = 0; b < kNBucket && e == hipSuccess; ++b) {
+        clm_ctx::Bin& bn = c->bins[b];
+        if (bn.n == 0) continue;
+        int bestG = bn.G, bestR = bn.R;
+        float best = 1e30f;
+        for (int i = 0; i < kNMenu && e == hipSuccess; ++i) {
+            int cap = kMenu[i].G * kMenu[i].R;
+            if (cap < c->bucket_nmax[b] || cap > 2 * kBucketMax[b]) continue;
+            bn.G = kMenu[i].G;
+            bn.R = kMenu[i].R;
+            float tmin = 1e30f;
+            for (int k = 0; k < reps && e == hipSuccess; ++k) {
+                e = copy_all(false);
+                if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+                if (e == hipSuccess) {
+                    if (c->jac_prec == 32)
+                        launch_bin_any<float>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters);
+                    else
+                        launch_bin_any<double>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters);
+                    e = hipGetLastError();
+                }
+                if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+                tmin = std::min(tmin, ms);
+            }
+            if (tmin < best) {
+                best = tmin;
+                bestG = bn.G;
+                bestR = bn.R;
+            }
+        }
+        bn.G = bestG;
+        bn.R = bestR;
+    }
+    if (e == hipSuccess) e = copy_all(false);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(save);
+    destroy_graph(c);
+    if (e != hipSuccess)
+        return set_err(c, CLM_E_HIP, "autotune failed: %s", hipGetErrorString(e));
     return CLM_OK;
 }
 

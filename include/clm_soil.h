@@ -203,6 +203,21 @@ int clm_run(clm_ctx* ctx, double t0, double dt, int32_t nsteps, int32_t steps_pe
  * energy_hydrology.jl:1282-1320).  energy may be NULL. */
 int clm_totals(clm_ctx* ctx, double* water, double* energy);
 
+/* ---- group-solver shapes (CLM_SOLVER_GROUP) --------------------------------
+ * Columns are bucketed by depth (max layers 8, 16, 24, 32, 40, 48, 64, 80,
+ * 96, 128); each bucket runs with one shape: G lanes per column, R rows per
+ * lane, G*R >= the bucket's deepest column.  Records are
+ * (bucket max layers, G, R, columns in bucket). */
+int clm_get_shapes(clm_ctx* ctx, int32_t* records /* 4 x max_records */,
+                   int32_t max_records, int32_t* nrecords);
+/* Force a bucket's shape; (G, R) must be one of the compiled shapes. */
+int clm_set_shape(clm_ctx* ctx, int32_t bucket_max_layers, int32_t G, int32_t R);
+/* Autotuner: for every non-empty bucket, time each compiled shape that fits
+ * (reps launches of one step from the current state and clm_set_forcing
+ * values; the state is restored afterwards) and keep the fastest.  Invalidates
+ * the captured run graph.  Allocates a temporary state copy. */
+int clm_autotune(clm_ctx* ctx, double dt, int32_t reps);
+
 /* Device time of the dominant kernel in the last clm_step/clm_run, per
  * launch (ms), measured with HIP events on the ctx stream. */
 int clm_last_kernel_ms(const clm_ctx* ctx, double* ms_per_launch, int32_t* launches);
