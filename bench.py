@@ -53,7 +53,17 @@ def cpu_baseline(batch, nsteps: int, ncol_sample: int, threads: int):
     return ncol_sample * nsteps / el, el
 
 
-def run_form(batch, jac, steps, warmup, solver=0):
+def parse_shapes(spec):
+    """"16:16x1,24:32x1" -> [(16, 16, 1), (24, 32, 1)] (bucket max layers: G x R)."""
+    out = []
+    for item in filter(None, (spec or "").split(",")):
+        mx, gr = item.split(":")
+        g, r = gr.split("x")
+        out.append((int(mx), int(g), int(r)))
+    return out
+
+
+def run_form(batch, jac, steps, warmup, solver=0, shapes_spec=None):
     import torch
     from clmsoil.model import LandSimulation, model_from_batch
     w = batch.workload
@@ -62,13 +72,18 @@ def run_form(batch, jac, steps, warmup, solver=0):
                          tol=w.newton_tol)
     sim.set_forcing_table(batch.top_water, batch.top_heat)
     shapes = None
-    if m.shapes():  # group solver: autotune the per-depth shapes on the initial state
+    if m.shapes() and shapes_spec:  # group solver, shapes given
+        for mx, g, r in parse_shapes(shapes_spec):
+            m.set_shape(mx, g, r)
+        shapes = [s for s in m.shapes() if s[3] > 0]
+    elif m.shapes():  # group solver: autotune the per-depth shapes on the initial state
         m.boundary_conditions["top"] = _top_bc(batch)
         m.set_forcing()
         sim._push()
         m.autotune(w.dt, reps=2)
         shapes = [s for s in m.shapes() if s[3] > 0]
-        print(f"[bench] autotuned shapes (max layers, lanes/column, rows/lane, columns): "
+    if shapes:
+        print(f"[bench] shapes (max layers, lanes/column, rows/lane, columns): "
               f"{shapes}", file=sys.stderr, flush=True)
     if warmup > 0:  # untimed warmup steps
         sim.solve(None, None, warmup, w.steps_per_slot, upload=False)
@@ -108,6 +123,13 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solver", type=int, default=0, help="0 auto, 1 group, 2 lane")
+    ap.add_argument("--shapes", default=None,
+                    help='fix group shapes instead of autotuning, e.g. "16:16x1,24:32x1"')
+    ap.add_argument("--only", action="store_true",
+                    help="run the chosen form only (no plain FP64 comparison)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, run a plain device copy of MIB MiB 5 times "
+                         "(reference kernel for HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; columns shard across GPUs with no "
@@ -125,15 +147,25 @@ def main():
     chosen = forms[args.form]
     res = {}
     for name, jac in (("fp64", 64), ("mixed", 32)):
-        if batch.model == 0 and name == "mixed" and args.form == "fp64":
+        if name != args.form and (args.only or (batch.model == 0 and args.form == "fp64")):
             continue
-        r = run_form(batch_steps, jac, steps, args.warmup, args.solver)
+        r = run_form(batch_steps, jac, steps, args.warmup, args.solver, args.shapes)
         res[name] = r
         st = r["stats"]
         print(f"[bench] {name}: {batch.ncol * steps / r['wall']:.4g} column-steps/s, "
               f"{r['wall'] * 1e3 / steps:.3f} ms/step, mean Newton iters {st.mean_iters:.2f}, "
               f"non-converged {st.n_nonconverged}, non-finite {st.n_nan}",
               file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: 5 copies of {n * 8} bytes", file=sys.stderr,
+              flush=True)
     r = res[args.form]
     st = r["stats"]
     value = batch.ncol * steps / r["wall"]
@@ -159,8 +191,10 @@ def main():
                                         else ""),
                    "solver": "group" if r.get("shapes") else "lane",
                    "shapes": r.get("shapes")},
-        "plain_fp64_value": batch.ncol * steps / res["fp64"]["wall"],
-        "speedup_vs_plain_fp64": value / (batch.ncol * steps / res["fp64"]["wall"]),
+        "plain_fp64_value": (batch.ncol * steps / res["fp64"]["wall"]) if "fp64" in res
+        else None,
+        "speedup_vs_plain_fp64": (value / (batch.ncol * steps / res["fp64"]["wall"]))
+        if "fp64" in res else None,
         "newton": {"mean_iters": st.mean_iters, "max_iters": st.max_iters_used,
                    "nonconverged_column_steps": st.n_nonconverged,
                    "nonfinite_column_steps": st.n_nan,

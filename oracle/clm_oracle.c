@@ -592,7 +592,7 @@ static double* scratch_get(col_scratch* s, int need) {
 
 static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_top,
                         double f_bot, double dt, int max_iters, double tol, int* iters,
-                        col_scratch* S) {
+                        col_scratch* S, const double* gv) {
     int32_t a = m->col_ptr[c];
     int N = m->col_ptr[c + 1] - a;
     col_geom g = {N, m->z_face + a + c};
@@ -610,6 +610,10 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
     memcpy(U0, Y->vl, sizeof(double) * N);
     double U0iw = *Y->iw;
     double* x = Y->vl;
+    if (gv) { /* warm start x0 = U0 + guess; cache_imp!(x0), BC fluxes stay at u_n */
+        for (int i = 0; i < N; ++i) x[i] += gv[i];
+        rre_aux(&g, &q, x, K, psi);
+    }
     int k;
     for (k = 1; k <= max_iters; ++k) {
         /* Wfact(x) */
@@ -647,7 +651,8 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
 
 static int eh_step_col(const orc_model* m, int64_t c, col_state* Y, double f_top_w,
                        double f_top_h, double f_bot_w, double f_bot_h, double dt,
-                       int max_iters, double tol, int* iters, col_scratch* S, int* perr) {
+                       int max_iters, double tol, int* iters, col_scratch* S, int* perr,
+                       const double* gv, const double* ge) {
     int32_t a = m->col_ptr[c];
     int N = m->col_ptr[c + 1] - a;
     col_geom g = {N, m->z_face + a + c};
@@ -677,6 +682,13 @@ static int eh_step_col(const orc_model* m, int64_t c, col_state* Y, double f_top
     memcpy(Y->re, U0e, sizeof(double) * N);
     *Y->iw = U0w;
     *Y->ie = U0e_s;
+    if (gv || ge) { /* warm start x0 = U0 + guess; cache_imp!(x0): T, ψ at x0 */
+        for (int i = 0; i < N; ++i) {
+            if (gv) Y->vl[i] += gv[i];
+            if (ge) Y->re[i] += ge[i];
+        }
+        eh_imp_aux(&g, &q, Y->vl, Y->ti, Y->re, T, psi);
+    }
     int k;
     for (k = 1; k <= max_iters; ++k) {
         eh_jacobian(&g, &q, Y->vl, Y->ti, K, T, kap, thl, dt, Www, Wee, Wew, tmpD, tmpK);
@@ -893,6 +905,12 @@ int orc_solve(const orc_model* m, const double* W_ww, const double* W_ee,
 
 int64_t orc_step(const orc_model* m, orc_state* Y, const orc_forcing* f, double dt,
                  int max_iters, double tol, int32_t* iters_out) {
+    return orc_step_guess(m, Y, f, dt, max_iters, tol, iters_out, NULL, NULL);
+}
+
+int64_t orc_step_guess(const orc_model* m, orc_state* Y, const orc_forcing* f, double dt,
+                       int max_iters, double tol, int32_t* iters_out, const double* guess_v,
+                       const double* guess_e) {
     int64_t nonconv = 0;
     int err = 0;
 #pragma omp parallel reduction(+ : nonconv) reduction(| : err)
@@ -903,13 +921,16 @@ int64_t orc_step(const orc_model* m, orc_state* Y, const orc_forcing* f, double 
             col_state y;
             view_state(m, Y, c, &y);
             int it = 0;
+            int32_t o = m->col_ptr[c];
+            const double* gv = guess_v ? guess_v + o : NULL;
+            const double* ge = guess_e ? guess_e + o : NULL;
             if (m->model == ORC_MODEL_RICHARDS)
                 nonconv += rre_step_col(m, c, &y, fval(f->top_water, c), fval(f->bot_water, c),
-                                        dt, max_iters, tol, &it, &S);
+                                        dt, max_iters, tol, &it, &S, gv);
             else
                 nonconv += eh_step_col(m, c, &y, fval(f->top_water, c), fval(f->top_heat, c),
                                        fval(f->bot_water, c), fval(f->bot_heat, c), dt,
-                                       max_iters, tol, &it, &S, &err);
+                                       max_iters, tol, &it, &S, &err, gv, ge);
             if (iters_out) iters_out[c] = it;
         }
         free(S.buf);
@@ -944,10 +965,11 @@ int64_t orc_run(const orc_model* m, orc_state* Y, const double* top_water_tab,
                 double fh = top_heat_tab ? top_heat_tab[(int64_t)slot * m->ncol + c] : 0.0;
                 int it = 0;
                 if (m->model == ORC_MODEL_RICHARDS)
-                    nonconv += rre_step_col(m, c, &y, fw, 0.0, dt, max_iters, tol, &it, &S);
+                    nonconv += rre_step_col(m, c, &y, fw, 0.0, dt, max_iters, tol, &it, &S,
+                                            NULL);
                 else
                     nonconv += eh_step_col(m, c, &y, fw, fh, 0.0, 0.0, dt, max_iters, tol, &it,
-                                           &S, &err);
+                                           &S, &err, NULL, NULL);
                 if (iters_out) iters_out[(int64_t)s * m->ncol + c] = it;
             }
         }

@@ -148,6 +148,14 @@ int64_t orc_step(const orc_model* m, orc_state* Y, const orc_forcing* f, double 
 /* nsteps steps; forcing slot s = (step / steps_per_slot) % nslots, slot
  * arrays laid out [slot][ncol].  iters_out is [nsteps][ncol] or NULL.
  * ncol_limit: only columns [0, ncol_limit) are advanced (sampling). */
+/* orc_step with a Newton warm start: x0 = U0 + guess (guess_v on ϑ_l,
+ * guess_e on ρe_int, per cell; either may be NULL), the implicit cache is
+ * refreshed at x0, boundary fluxes stay those of u_n.  With both NULL this
+ * is orc_step. */
+int64_t orc_step_guess(const orc_model* m, orc_state* Y, const orc_forcing* f, double dt,
+                       int max_iters, double tol, int32_t* iters_out, const double* guess_v,
+                       const double* guess_e);
+
 int64_t orc_run(const orc_model* m, orc_state* Y, const double* top_water_tab,
                 const double* top_heat_tab, int nslots, int steps_per_slot,
                 double dt, int nsteps, int max_iters, double tol,

@@ -104,6 +104,8 @@ def lib(variant: str = ""):
         L.orc_solve.argtypes = [M, _dp, _dp, _dp, S, S]
         L.orc_step.argtypes = [M, S, F, d, C.c_int, d, _ip]
         L.orc_step.restype = C.c_int64
+        L.orc_step_guess.argtypes = [M, S, F, d, C.c_int, d, _ip, _dp, _dp]
+        L.orc_step_guess.restype = C.c_int64
         L.orc_run.argtypes = [M, S, _dp, _dp, C.c_int, C.c_int, d, C.c_int, C.c_int, d, _ip,
                               C.c_int64, C.c_int]
         L.orc_run.restype = C.c_int64
@@ -211,11 +213,14 @@ class OracleModel:
         return dx
 
     def step(self, Y: dict, dt: float, max_iters: int, tol: float, tw, th=None, bw=None,
-             bh=None):
+             bh=None, guess_v=None, guess_e=None):
+        """One ARS111 step in place; guess_v / guess_e (per cell) warm-start
+        Newton at x0 = U0 + guess."""
         it = np.zeros(self.ncol, dtype=np.int32)
         f = self._f(tw, th, bw, bh)
-        nc = self.L.orc_step(C.byref(self._m), C.byref(self._s(Y)), C.byref(f), dt, max_iters,
-                            tol, it.ctypes.data_as(_ip))
+        nc = self.L.orc_step_guess(C.byref(self._m), C.byref(self._s(Y)), C.byref(f), dt,
+                                  max_iters, tol, it.ctypes.data_as(_ip), _p(guess_v),
+                                  _p(guess_e))
         if nc < 0:
             raise ValueError("Matric potential is positive")
         return it, int(nc)

@@ -361,3 +361,57 @@ def test_converged_step_is_root_of_implicit_euler(oracle_mod, name):
             assert np.allclose(Ys["rho_e_int"], z[N:] * 1e6, rtol=1e-9, atol=1e-3), (name, c)
         checked += 1
     assert checked >= 3
+
+
+@pytest.mark.parametrize("name", ["rre_131072_uneven", "eh_262144_uneven"])
+def test_warm_start_reaches_the_same_root(oracle_mod, name):
+    """Newton from x0 = U0 + guess: the exact increment converges at once; a
+    perturbed guess ends at the plain step's root to the Newton tolerance
+    (Newton stays the judge).  For RichardsModel a zero guess is the plain
+    step bit for bit; for EnergyHydrology it is not, because the warm start
+    refreshes T and ψ at x0 while the plain first iteration uses those of
+    u_n -- it still reaches the same root."""
+    from clmsoil import inputs
+    b = inputs.make_batch(name, ncol=256)
+    w = b.workload
+    o = oracle_mod.OracleModel.from_batch(b)
+    fw = b.top_water[0].copy()
+    fh = None if b.top_heat is None else b.top_heat[0].copy()
+    eh = b.model == 1
+    U = b.copy_state()
+    o.step(U, w.dt, 0, w.newton_tol, fw, fh)  # zero iterations: the predictor U0
+    X = b.copy_state()
+    it_x, _ = o.step(X, w.dt, 200, 1e-13, fw, fh)
+    P = b.copy_state()
+    it_p, _ = o.step(P, w.dt, w.newton_max_iters, w.newton_tol, fw, fh)
+    Z = b.copy_state()
+    zero = np.zeros(b.ncell)
+    it_z, _ = o.step(Z, w.dt, w.newton_max_iters, w.newton_tol, fw, fh, guess_v=zero,
+                     guess_e=zero if eh else None)
+    if not eh:
+        assert np.array_equal(it_z, it_p)
+        for k in P:
+            assert np.array_equal(np.nan_to_num(Z[k]), np.nan_to_num(P[k])), k
+    good = (it_x < 200) & (it_p < w.newton_max_iters)
+    good &= np.isfinite(np.add.reduceat(X["vartheta_l"], b.col_ptr[:-1]))
+    assert good.mean() > 0.5
+    cm = np.repeat(good, b.nlayers)
+    gv = np.where(cm, X["vartheta_l"] - U["vartheta_l"], 0.0)
+    ge = np.where(cm, X["rho_e_int"] - U["rho_e_int"], 0.0) if eh else None
+    E = b.copy_state()
+    it_e, _ = o.step(E, w.dt, w.newton_max_iters, w.newton_tol, fw, fh, guess_v=gv, guess_e=ge)
+    assert (it_e[good] <= 2).all()
+    assert np.abs(E["vartheta_l"] - X["vartheta_l"])[cm].max() <= 1e-9
+    rng = np.random.default_rng(3)
+    Q = b.copy_state()
+    it_q, _ = o.step(Q, w.dt, w.newton_max_iters, w.newton_tol, fw, fh,
+                     guess_v=gv * (1 + 0.1 * rng.standard_normal(b.ncell)),
+                     guess_e=None if ge is None else ge * (1 + 0.1 * rng.standard_normal(b.ncell)))
+    both = good & (it_q < w.newton_max_iters)
+    cb = np.repeat(both, b.nlayers)
+    assert both.mean() > 0.5
+    # two Newton solutions to tolerance 1e-8 from different starts agree to ~ the tolerance
+    assert np.abs(Q["vartheta_l"] - X["vartheta_l"])[cb].max() <= 1e-7
+    assert np.abs(P["vartheta_l"] - X["vartheta_l"])[cb].max() <= 1e-7
+    cz = np.repeat(good & (it_z < w.newton_max_iters), b.nlayers)
+    assert np.abs(Z["vartheta_l"] - X["vartheta_l"])[cz].max() <= 1e-7
