@@ -7,9 +7,13 @@ workload's tolerance) over every column of the workload.  Inputs are
 synthetic (clmsoil.inputs) and resident in HBM before timing; the K steps
 run as one captured HIP graph.  Prints one JSON line.
 
-Forms: "mixed" = FP32 Jacobian + FP32 solve inside an FP64-residual Newton;
+Forms: "learned" = the learned Newton warm start on top of "mixed";
+"mixed" = FP32 Jacobian + FP32 solve inside an FP64-residual Newton;
 "fp64" = the plain FP64 path.  `value` is the chosen form's throughput; the
-plain FP64 throughput on the same workload is reported beside it.
+plain FP64 throughput on the same workload is reported beside it.  The
+learned form's network is trained first (untimed) on columns drawn from other
+seeds of the same workload: 8 plain steps of all its columns give the
+(state, converged step) pairs.
 """
 import argparse
 import json
@@ -26,18 +30,38 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def design_bytes(batch, iters: np.ndarray, jac_precision: int) -> float:
-    """HBM bytes the fused step kernel moves for the given per-column-step
-    Newton iteration counts ([steps][ncol]).  Per-level figures follow the
-    loads/stores of k_rre_step / k_eh_step."""
-    j = 4 if jac_precision == 32 else 8
+def step_bytes(batch, learned: bool) -> float:
+    """Compulsory HBM bytes of one fused step over all columns (the group
+    kernels read the column state, faces, parameters and forcing once and
+    write the state once; everything else stays in registers).  Per column:
+    EnergyHydrology 56N + 176, RichardsModel 24N + 96; the learned form adds
+    the per-cell guess the step kernel reads (16N EH, 8N RRE)."""
+    N = batch.nlayers.astype(np.float64)
+    if batch.model == 1:
+        per = 56 * N + 176 + (16 * N if learned else 0)
+    else:
+        per = 24 * N + 96 + (8 * N if learned else 0)
+    return float(per.sum())
+
+
+def unfused_bytes(batch, iters: np.ndarray) -> float:
+    """Bytes the unfused hook-by-hook decomposition would move for the same
+    Newton iteration counts ([steps][ncol]): explicit stage plus, per
+    iteration, assemble (aux + residual + Jacobian) and solve, each streaming
+    its operands through HBM (EH 80N+136 + k(280N+96); RRE k(104N+72)+64)."""
     N = batch.nlayers.astype(np.float64)[None, :]
     k = (iters & 0xFFFF).astype(np.float64)
-    if batch.model == 0:
-        per_col = N * 8 + k * N * (40 + 4 * j) + 80.0
+    if batch.model == 1:
+        per = 80 * N + 136 + k * (280 * N + 96)
     else:
-        per_col = N * (32 + 72 + j) + k * N * (208 + 13 * j) + 152.0
-    return float(per_col.sum())
+        per = k * (104 * N + 72) + 64
+    return float(per.sum())
+
+
+def mlp_flops(batch) -> float:
+    """Multiply-add FLOPs of one warm-start inference over all columns."""
+    F, Fp, H, No = (105, 128, 384, 64) if batch.model == 1 else (40, 64, 256, 32)
+    return 2.0 * batch.ncol * (Fp * H + 2 * H * H + H * No)
 
 
 def cpu_baseline(batch, nsteps: int, ncol_sample: int, threads: int):
@@ -63,11 +87,44 @@ def parse_shapes(spec):
     return out
 
 
-def run_form(batch, jac, steps, warmup, solver=0, shapes_spec=None):
+def train_warm_start(name, ncol, jac, epochs, collect_steps=8, batch=8192):
+    """Collect (state, converged step) pairs with the plain solver on columns
+    of other seeds and train the warm-start network on the device."""
     import torch
-    from clmsoil.model import LandSimulation, model_from_batch
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
+    bt = inputs.make_batch(inputs.training_workload(name), ncol=ncol, nsteps=collect_steps)
+    w = bt.workload
+    m = model_from_batch(bt, jac_precision=jac, solver=1)
+    ws = NewtonWarmStart(m, seed=5)
+    sim = LandSimulation(m, bt.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                         tol=w.newton_tol)
+    t0 = time.perf_counter()
+    rows = ws.collect(sim, bt.top_water, bt.top_heat, collect_steps, w.steps_per_slot)
+    t1 = time.perf_counter()
+    losses = ws.train(epochs, batch=batch, lr=1e-3, seed=9)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    params = ws.params()
+    m.close()
+    info = {"pairs": rows, "epochs": epochs, "batch": batch, "params": int(params.size),
+            "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+            "collect_s": t1 - t0, "train_s": t2 - t1}
+    print(f"[bench] warm start: {rows} pairs collected in {t1 - t0:.1f} s, {epochs} epochs in "
+          f"{t2 - t1:.1f} s, loss {losses[0]:.4g} -> {losses[-1]:.4g}", file=sys.stderr,
+          flush=True)
+    return params, info
+
+
+def run_form(batch, jac, steps, warmup, solver=0, shapes_spec=None, ws_params=None):
+    import torch
+    from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
     w = batch.workload
     m = model_from_batch(batch, jac_precision=jac, solver=solver)
+    if ws_params is not None:
+        ws = NewtonWarmStart(m)
+        ws.set_params(ws_params)
+        ws.enable(True)
     sim = LandSimulation(m, batch.copy_state(), w.dt, max_iters=w.newton_max_iters,
                          tol=w.newton_tol)
     sim.set_forcing_table(batch.top_water, batch.top_heat)
@@ -100,9 +157,17 @@ def run_form(batch, jac, steps, warmup, solver=0, shapes_spec=None):
     # the same (deterministic) run again to copy out its iteration table
     sim.reset(batch.copy_state())
     _, iters = sim.solve(None, None, steps, w.steps_per_slot, want_iters=True, upload=False)
-    kernel_ms = st.ms / steps
+    graph_ms = st.ms / steps
+    infer_ms = 0.0
+    if ws_params is not None:  # device time of the inference part of each step
+        sim.reset(batch.copy_state())
+        sim._push()
+        m.boundary_conditions["top"] = _top_bc(batch)
+        m.set_forcing()
+        infer_ms = ws.inference_ms(w.dt, reps=5)
     m.close()
-    return dict(wall=wall, stats=st, iters=iters, kernel_ms=kernel_ms, shapes=shapes)
+    return dict(wall=wall, stats=st, iters=iters, graph_ms=graph_ms, infer_ms=infer_ms,
+                kernel_ms=graph_ms - infer_ms, shapes=shapes)
 
 
 def _top_bc(batch):
@@ -118,7 +183,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="eh_262144_uneven")
-    ap.add_argument("--form", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--form", default="learned", choices=["learned", "mixed", "fp64"])
+    ap.add_argument("--epochs", type=int, default=8, help="warm-start training epochs")
     ap.add_argument("--ncol", type=int, default=None, help="prefix of the workload (testing)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -127,6 +193,9 @@ def main():
                     help='fix group shapes instead of autotuning, e.g. "16:16x1,24:32x1"')
     ap.add_argument("--only", action="store_true",
                     help="run the chosen form only (no plain FP64 comparison)")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per step measured with rocprofv3 (FETCH_SIZE, WRITE_SIZE, "
+                         "scaled by a copy kernel) to report beside the compulsory bytes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed steps, run a plain device copy of MIB MiB 5 times "
                          "(reference kernel for HBM traffic counters)")
@@ -143,13 +212,21 @@ def main():
         if steps != w.nsteps else batch
     print(f"[bench] workload {w.name}: {batch.ncol} columns, {batch.ncell} cells, {steps} "
           f"steps", file=sys.stderr, flush=True)
-    forms = {"mixed": 32, "fp64": 64}
+    forms = {"learned": 32, "mixed": 32, "fp64": 64}
+    if batch.model == 0:  # RichardsModel: the learned form runs on the FP64 Jacobian
+        forms["learned"] = 64
     chosen = forms[args.form]
     res = {}
-    for name, jac in (("fp64", 64), ("mixed", 32)):
-        if name != args.form and (args.only or (batch.model == 0 and args.form == "fp64")):
+    ws_params, ws_info = None, None
+    if args.form == "learned":
+        ws_params, ws_info = train_warm_start(w.name, batch.ncol, chosen, args.epochs)
+    for name, jac in (("fp64", 64), ("mixed", 32), ("learned", forms["learned"])):
+        if name != args.form and (args.only or (batch.model == 0 and name == "mixed")):
             continue
-        r = run_form(batch_steps, jac, steps, args.warmup, args.solver, args.shapes)
+        if name == "learned" and ws_params is None:
+            continue
+        r = run_form(batch_steps, jac, steps, args.warmup, args.solver, args.shapes,
+                     ws_params if name == "learned" else None)
         res[name] = r
         st = r["stats"]
         print(f"[bench] {name}: {batch.ncol * steps / r['wall']:.4g} column-steps/s, "
@@ -169,8 +246,11 @@ def main():
     r = res[args.form]
     st = r["stats"]
     value = batch.ncol * steps / r["wall"]
-    bytes_total = design_bytes(batch_steps, r["iters"], chosen)
-    achieved = bytes_total / steps / (r["kernel_ms"] * 1e-3) / 1e9
+    learned = args.form == "learned"
+    bytes_step = step_bytes(batch_steps, learned)
+    achieved = bytes_step / (r["kernel_ms"] * 1e-3) / 1e9
+    unfused = unfused_bytes(batch_steps, r["iters"]) / steps
+    kname = "k_eh_coop" if batch.model == 1 else "k_rre_coop"
     out = {
         "metric": "column-steps/s at Newton residual 1e-8: learned+mixed vs plain FP64 GPU",
         "value": value,
@@ -182,32 +262,49 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if chosen == 64 else "f64 residual/state, f32 jacobian+solve",
+        "dtype": ("f64" if chosen == 64 else "f64 residual/state, f32 jacobian+solve") +
+                 (", bf16 warm-start network (fp32 accumulate)" if args.form == "learned" else ""),
         "data": "synthetic",
         "config": {"workload": w.name, "columns": batch.ncol, "layers": list(w.layers),
                    "dt_s": w.dt, "newton_tol": w.newton_tol,
                    "newton_max_iters": w.newton_max_iters,
-                   "form": args.form + (" (no learned warm start yet)" if args.form == "mixed"
-                                        else ""),
+                   "form": {"learned": "learned warm start + mixed precision",
+                            "mixed": "mixed precision (FP32 Jacobian + solve)",
+                            "fp64": "plain FP64"}[args.form],
                    "solver": "group" if r.get("shapes") else "lane",
                    "shapes": r.get("shapes")},
         "plain_fp64_value": (batch.ncol * steps / res["fp64"]["wall"]) if "fp64" in res
         else None,
         "speedup_vs_plain_fp64": (value / (batch.ncol * steps / res["fp64"]["wall"]))
         if "fp64" in res else None,
+        "other_forms": {k: {"value": batch.ncol * steps / v["wall"],
+                            "ms_per_step": v["wall"] * 1e3 / steps,
+                            "mean_newton_iters": v["stats"].mean_iters}
+                        for k, v in res.items()},
+        "warm_start_training": ws_info,
+        "warm_start_inference": None if not learned else {
+            "ms_per_step": r["infer_ms"],
+            "gemm_tflops": mlp_flops(batch_steps) / (r["infer_ms"] * 1e-3) / 1e12,
+            "mfma_bf16_peak_tflops": 2500.0},
         "newton": {"mean_iters": st.mean_iters, "max_iters": st.max_iters_used,
                    "nonconverged_column_steps": st.n_nonconverged,
                    "nonfinite_column_steps": st.n_nan,
                    "converged_fraction": 1.0 - (st.n_nonconverged / (batch.ncol * steps))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_eh_step" if batch.model == 1 else "k_rre_step",
-                     "kernel_ms_per_launch": r["kernel_ms"],
-                     "bytes_per_launch": bytes_total / steps},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": args.traffic,
+                     "kernel": kname + " (one launch per depth bin; per step)",
+                     "kernel_ms_per_step": r["kernel_ms"],
+                     "bytes_per_step": bytes_step,
+                     "note": "fused step: compulsory bytes only; the kernel is bound by FP64 "
+                             "VALU issue, not HBM",
+                     "unfused_equivalent_GBps": unfused / (r["kernel_ms"] * 1e-3) / 1e9,
+                     "unfused_equivalent_frac": unfused / (r["kernel_ms"] * 1e-3) / 1e9
+                     / HBM_PEAK_GBS},
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        sample = args.cpu_sample or min(batch.ncol, 2048 if batch.model == 1 else 4096)
+        sample = args.cpu_sample or min(batch.ncol, 24576 if batch.model == 1 else 16384)
         cps, el = cpu_baseline(batch_steps, steps, sample, threads)
         out["cpu_baseline"] = {"value": cps, "unit": "column-steps/s", "cores": threads,
                                "kind": "port",

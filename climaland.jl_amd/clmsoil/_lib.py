@@ -28,10 +28,14 @@ EXPORTED = (
     "clm_exp_tendency", "clm_imp_tendency", "clm_jacobian", "clm_solve",
     "clm_update_implicit_cache", "clm_get_aux", "clm_get_jacobian", "clm_step", "clm_run",
     "clm_totals", "clm_last_kernel_ms", "clm_get_shapes", "clm_set_shape", "clm_autotune",
+    "clm_ws_init", "clm_ws_dims", "clm_ws_get_params", "clm_ws_set_params", "clm_ws_enable",
+    "clm_ws_collect", "clm_ws_train", "clm_ws_features", "clm_ws_forward", "clm_ws_gradient",
+    "clm_ws_time",
 )
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
+_fp = C.POINTER(C.c_float)
 
 
 class Config(C.Structure):
@@ -106,6 +110,18 @@ def load(path: str | None = None) -> C.CDLL:
         "clm_get_shapes": ([vp, _ip, C.c_int32, _ip], C.c_int),
         "clm_set_shape": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),
         "clm_autotune": ([vp, C.c_double, C.c_int32], C.c_int),
+        "clm_ws_init": ([vp, C.c_uint64], C.c_int),
+        "clm_ws_dims": ([vp, _ip, C.POINTER(C.c_int64)], C.c_int),
+        "clm_ws_get_params": ([vp, _fp], C.c_int),
+        "clm_ws_set_params": ([vp, _fp], C.c_int),
+        "clm_ws_enable": ([vp, C.c_int32], C.c_int),
+        "clm_ws_collect": ([vp, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int64)],
+                           C.c_int),
+        "clm_ws_train": ([vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp], C.c_int),
+        "clm_ws_features": ([vp, C.c_double, _fp], C.c_int),
+        "clm_ws_forward": ([vp, _fp, C.c_int64, _fp], C.c_int),
+        "clm_ws_gradient": ([vp, _fp, _fp, _fp, C.c_int64, _fp, _fp], C.c_int),
+        "clm_ws_time": ([vp, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -120,6 +136,13 @@ def dptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "float64 C-contiguous required"
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "float32 C-contiguous required"
+    return a.ctypes.data_as(_fp)
 
 
 def iptr(a: np.ndarray | None):

@@ -183,6 +183,16 @@ def kappa_solid(nu_om, nu_q):
     return KAPPA_OM ** nu_om * KAPPA_QUARTZ ** nu_q * KAPPA_MINERALS ** (1.0 - nu_om - nu_q)
 
 
+def training_workload(name: str) -> str:
+    """Name of a workload drawn like `name` from other seeds (config_id + 1000):
+    columns for training the warm start that the evaluation never sees."""
+    tname = name + "_train"
+    if tname not in WORKLOADS:
+        w = WORKLOADS[name]
+        WORKLOADS[tname] = dataclasses.replace(w, name=tname, config_id=w.config_id + 1000)
+    return tname
+
+
 def make_batch(name: str, ncol: int | None = None, nsteps: int | None = None) -> ColumnBatch:
     """The first `ncol` columns of workload `name` (default: all of them)."""
     w = WORKLOADS[name]

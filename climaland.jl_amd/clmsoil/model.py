@@ -327,6 +327,85 @@ def make_compute_jacobian(model: _SoilModel):
 # ---- simulation driver (Simulations.jl) ----------------------------------------
 
 
+class NewtonWarmStart:
+    """Learned Newton warm start of a soil model (clm_ws_* in clm_soil.h).
+
+    A per-column MLP predicts the implicit part of the step's increment;
+    Newton starts from U0 + guess and still iterates to the model's
+    tolerance.  The network is trained on the device from (state, converged
+    step) pairs that the plain solver produces (`collect`, then `train`).
+    It has no counterpart in the reference, where Newton starts at U0.
+    """
+
+    def __init__(self, model: "_SoilModel", seed: int = 0):
+        self.model = model
+        model._check(model._lib.clm_ws_init(model._ctx, seed))
+        d = np.zeros(4, dtype=np.int32)
+        n = C.c_int64(0)
+        model._check(model._lib.clm_ws_dims(model._ctx, L.iptr(d), C.byref(n)))
+        self.F, self.Fp, self.H, self.Nout = (int(x) for x in d)
+        self.n_params = int(n.value)
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, dtype=np.float32)
+        self.model._check(self.model._lib.clm_ws_get_params(self.model._ctx, L.fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        self.model._check(self.model._lib.clm_ws_set_params(self.model._ctx, L.fptr(p)))
+
+    def enable(self, on: bool = True):
+        self.model._check(self.model._lib.clm_ws_enable(self.model._ctx, 1 if on else 0))
+
+    def collect(self, sim: "LandSimulation", top_water, top_heat, nsteps: int,
+                steps_per_slot: int) -> int:
+        """Advance `sim` nsteps plain steps on the device, keeping the pairs."""
+        sim.set_forcing_table(top_water, top_heat)
+        sim._push()
+        rows = C.c_int64(0)
+        m = self.model
+        m._check(m._lib.clm_ws_collect(m._ctx, sim.Δt, nsteps, steps_per_slot, C.byref(rows)))
+        sim.t += nsteps * sim.Δt
+        return int(rows.value)
+
+    def train(self, epochs: int, batch: int = 8192, lr: float = 1e-3, seed: int = 1):
+        losses = np.zeros(epochs, dtype=np.float32)
+        m = self.model
+        m._check(m._lib.clm_ws_train(m._ctx, epochs, batch, lr, seed, L.fptr(losses)))
+        return losses
+
+    def features(self, dt: float) -> np.ndarray:
+        X = np.zeros((self.model.domain.ncol, self.Fp), dtype=np.float32)
+        self.model._check(self.model._lib.clm_ws_features(self.model._ctx, dt, L.fptr(X)))
+        return X
+
+    def forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], self.Nout), dtype=np.float32)
+        self.model._check(self.model._lib.clm_ws_forward(self.model._ctx, L.fptr(X), X.shape[0],
+                                                         L.fptr(Y)))
+        return Y
+
+    def inference_ms(self, dt: float, reps: int = 5) -> float:
+        """Device time of one inference (features, network, guess), in ms."""
+        ms = C.c_double(0.0)
+        self.model._check(self.model._lib.clm_ws_time(self.model._ctx, dt, reps, C.byref(ms)))
+        return ms.value
+
+    def gradient(self, X: np.ndarray, T: np.ndarray, mask: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        mask = np.ascontiguousarray(mask, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self.model._check(self.model._lib.clm_ws_gradient(
+            self.model._ctx, L.fptr(X), L.fptr(T), L.fptr(mask), X.shape[0], L.fptr(loss),
+            L.fptr(g)))
+        return float(loss[0]), g
+
+
 class LandSimulation:
     """ARS111 + NewtonsMethod stepping of a soil model.
 

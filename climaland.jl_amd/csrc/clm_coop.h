@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
     double* __restrict__ xg, double* __restrict__ iw, const double* __restrict__ f_top,
     const double* __restrict__ f_bot, int bc_bottom, double dt, int max_iters, double tol,
-    int32_t* __restrict__ iters) {
+    int32_t* __restrict__ iters, const double* __restrict__ gv, double* __restrict__ u0out) {
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
     const int gl = id.gl;
@@ -177,6 +177,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     const int64_t base = col_ptr[col];
     const int N = (int)(col_ptr[col + 1] - base);
     const int64_t fbase = base + col;
+    const bool warm = gv != nullptr;
     const VG v = make_vg(par[P_VG_ALPHA * ncol + col], par[P_VG_N * ncol + col],
                          par[P_THETA_R * ncol + col], par[P_SS * ncol + col],
                          par[P_KSAT * ncol + col]);
@@ -199,14 +200,34 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         u0[k] = xv;  // explicit tendency is zero (rre.jl:214-243): U0 = u_n
         zc[k] = 0.5 * (z0 + z1);
         dzc[k] = z1 - z0;
+        if (u0out && on[k]) u0out[base + r] = xv;
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);  // face below row
 
-    double bot = 0.0;
-    int kit = 0;
-    bool conv = false;
+    // lagged boundary fluxes from update_cache!(u_n)
+    double bot;
+    {
+        PsiD p0 = psi_dpsi(v, nu, u0[0]);
+        double K0 = on[0] ? hyd_cond_from_a(v, p0.S, p0.a) : 0.0;
+        double Kb = g_bcast<G>(K0, 0);
+        bot = (bc_bottom == BC_FREE_DRAINAGE) ? -1.0 * Kb : f_bot[col];
+    }
+    int kit = 0, kit_total = 0;
+    bool conv = false, fell_back = false;
     double err = 0.0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+    if (attempt == 0 && warm) {  // x0 = U0 + guess; cache_imp!(x0) happens at the loop top
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (on[k]) x[k] = u0[k] + gv[base + gl * R + k];
+    }
+    if (attempt == 1) {  // the warm start went non-finite: plain Newton from U0
+        if (!(warm && !isfinite(err))) break;
+        fell_back = true;
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = u0[k];
+    }
     for (kit = 1; kit <= max_iters; ++kit) {
         double K[R], h[R], D[R];
 #pragma unroll
@@ -216,8 +237,6 @@ __global__ __launch_bounds__(256) void k_rre_coop(
             h[k] = p.psi + zc[k];
             D[k] = p.D;
         }
-        if (kit == 1)  // lagged boundary fluxes from update_cache!(u_n)
-            bot = (bc_bottom == BC_FREE_DRAINAGE) ? -1.0 * g_bcast<G>(K[0], 0) : f_bot[col];
         // face below each row: conductance w and flux F
         double w[R], F[R];
 #pragma unroll
@@ -264,14 +283,17 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         if (conv || !isfinite(err)) break;
     }
     if (kit > max_iters) kit = max_iters;
+    kit_total += kit;
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k)
         if (on[k]) xg[base + gl * R + k] = x[k];
     if (gl == 0) {
         iw[col] = iw[col] + dt * (-(top - bot));  // ∫F: -I block, exact
-        int32_t flag = kit;
+        int32_t flag = kit_total;
         if (tol > 0.0 && !conv) flag |= 0x10000;
         if (!isfinite(err)) flag |= 0x20000;
+        if (fell_back) flag |= 0x80000;
         iters[col] = flag;
     }
 }
@@ -285,7 +307,8 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     double* __restrict__ iw, double* __restrict__ ie, const double* __restrict__ ftw,
     const double* __restrict__ fth, const double* __restrict__ fbw,
     const double* __restrict__ fbh, int bc_bottom, double dt, int max_iters, double tol,
-    int32_t* __restrict__ iters) {
+    int32_t* __restrict__ iters, const double* __restrict__ gv, const double* __restrict__ ge,
+    double* __restrict__ u0out) {
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
     const int gl = id.gl;
@@ -322,6 +345,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             F = phase_change(e.v, e.nu, a.thl, t, a.T, tau, &bad);
         }
         u0v[k] = v + dt * (-F);
+        if (u0out && on[k]) u0out[base + r] = u0v[k];
         xi[k] = t + dt * ((RHO_L / RHO_I) * F);
         u0e[k] = re;
         xv[k] = u0v[k];
@@ -356,9 +380,45 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         wk[k] = (r == 0) ? 0.0 : 0.5 * (km + kl[k]) / dzf[k];
     }
 
-    int kit;
-    bool conv = false;
+    const bool warm = gv != nullptr;
+    int kit = 0, kit_total = 0;
+    bool conv = false, fell_back = false;
     double err = 0.0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+    if (attempt == 0 && warm) {  // x0 = U0 + guess, cache_imp!(x0) and Wfact(x0)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            int r = gl * R + k;
+            if (on[k]) {
+                xv[k] = u0v[k] + gv[base + r];
+                xe[k] = u0e[k] + ge[base + r];
+            }
+            Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
+            PsiD pd = psi_dpsi(e.v, e.nu - xi[k], xv[k]);
+            Pp[k] = pd.psi;
+            D[k] = pd.D;
+        }
+    }
+    if (attempt == 1) {  // the warm start went non-finite: plain Newton from U0
+        if (!(warm && !isfinite(err))) break;
+        fell_back = true;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {  // the cache of u_n, re-read (HBM holds u_n until the end)
+            int r = gl * R + k;
+            double v0 = 0.3, t0 = 0.0, e0 = 0.0;
+            if (on[k]) {
+                v0 = xvg[base + r];
+                t0 = xig[base + r];
+                e0 = xeg[base + r];
+            }
+            EHAux a = eh_cell_aux(e, v0, t0, e0);
+            Tp[k] = a.T;
+            Pp[k] = a.psi;
+            xv[k] = u0v[k];
+            xe[k] = u0e[k];
+            D[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).D;
+        }
+    }
     for (kit = 1; kit <= max_iters; ++kit) {
         double h[R], E[R];
 #pragma unroll
@@ -460,6 +520,8 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         if (conv || !isfinite(err)) break;
     }
     if (kit > max_iters) kit = max_iters;
+    kit_total += kit;
+    }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (on[k]) {
@@ -473,10 +535,11 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     if (gl == 0) {
         iw[col] = iw[col] + dt * (-(topw - botw));
         ie[col] = ie[col] + dt * (-(toph - both));
-        int32_t flag = kit;
+        int32_t flag = kit_total;
         if (tol > 0.0 && !conv) flag |= 0x10000;
         if (!isfinite(err)) flag |= 0x20000;
         if (bad_g) flag |= 0x40000;
+        if (fell_back) flag |= 0x80000;
         iters[col] = flag;
     }
 }

@@ -15,6 +15,7 @@
 #include "clm_coop.h"
 #include "clm_eh.h"
 #include "clm_layout.h"
+#include "clm_mlp.h"
 #include "clm_rre.h"
 
 using namespace clm;
@@ -83,6 +84,7 @@ struct clm_ctx {
     };
     std::vector<Bin> bins;  // one per depth bucket (n may be 0)
     int bucket_G[16], bucket_R[16], bucket_nmax[16];
+    struct WsState* ws = nullptr;  // learned warm start (clm_ws_*)
     // stream, graph, events
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -392,58 +394,280 @@ void launch_step_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
 }
 
 // one launch per depth bin; every kernel is independent of the others
+// optional per-step extras of the group kernels: a Newton warm start
+// (x0 = U0 + guess) and an output copy of the predictor U0_ϑ
+struct StepExtra {
+    const double* gv = nullptr;
+    const double* ge = nullptr;
+    double* u0out = nullptr;
+};
+
 template <typename JT, int G, int R>
 void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const double* fth,
-                double dt, int32_t* iters) {
+                double dt, int32_t* iters, const StepExtra& x) {
     constexpr int gpb = 256 / G;  // columns per 256-thread block
     dim3 g((unsigned)((bn.n + gpb - 1) / gpb)), b(256);
     if (!is_eh(c)) {
         hipLaunchKernelGGL((k_rre_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
                            bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ciw, ftw, c->d_cfbw,
-                           c->bc_bottom, dt, c->max_iters, c->tol, iters);
+                           c->bc_bottom, dt, c->max_iters, c->tol, iters, x.gv, x.u0out);
     } else {
         hipLaunchKernelGGL((k_eh_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
                            bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci, c->d_ce,
                            c->d_ciw, c->d_cie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt,
-                           c->max_iters, c->tol, iters);
+                           c->max_iters, c->tol, iters, x.gv, x.ge, x.u0out);
     }
 }
 
 template <typename JT, int I>
 void launch_menu(clm_ctx* c, int idx, const clm_ctx::Bin& bn, const double* ftw,
-                 const double* fth, double dt, int32_t* iters) {
+                 const double* fth, double dt, int32_t* iters, const StepExtra& x) {
     if constexpr (I < kNMenu) {
         if (idx == I)
-            launch_bin<JT, kMenu[I].G, kMenu[I].R>(c, bn, ftw, fth, dt, iters);
+            launch_bin<JT, kMenu[I].G, kMenu[I].R>(c, bn, ftw, fth, dt, iters, x);
         else
-            launch_menu<JT, I + 1>(c, idx, bn, ftw, fth, dt, iters);
+            launch_menu<JT, I + 1>(c, idx, bn, ftw, fth, dt, iters, x);
     }
 }
 
 template <typename JT>
 void launch_bin_any(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const double* fth,
-                    double dt, int32_t* iters) {
-    if (bn.n > 0) launch_menu<JT, 0>(c, menu_index(bn.G, bn.R), bn, ftw, fth, dt, iters);
+                    double dt, int32_t* iters, const StepExtra& x = StepExtra()) {
+    if (bn.n > 0) launch_menu<JT, 0>(c, menu_index(bn.G, bn.R), bn, ftw, fth, dt, iters, x);
 }
 
 template <typename JT>
 void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
-                    int32_t* iters) {
-    for (const auto& bn : c->bins) launch_bin_any<JT>(c, bn, ftw, fth, dt, iters);
+                    int32_t* iters, const StepExtra& x) {
+    for (const auto& bn : c->bins) launch_bin_any<JT>(c, bn, ftw, fth, dt, iters, x);
 }
 
-void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters) {
+void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
+                     StepExtra* x);
+
+void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters,
+                 double* u0out = nullptr, bool allow_warm = true) {
     if (is_group(c)) {
+        StepExtra x;
+        x.u0out = u0out;
+        if (allow_warm) ws_launch_guess(c, ftw, fth, dt, &x);
         if (c->jac_prec == 32)
-            launch_group_t<float>(c, ftw, fth, dt, iters);
+            launch_group_t<float>(c, ftw, fth, dt, iters, x);
         else
-            launch_group_t<double>(c, ftw, fth, dt, iters);
+            launch_group_t<double>(c, ftw, fth, dt, iters, x);
         return;
     }
     if (c->jac_prec == 32)
         launch_step_t<float>(c, ftw, fth, dt, iters);
     else
         launch_step_t<double>(c, ftw, fth, dt, iters);
+}
+
+// ---- learned warm start (clm_ws_*) -----------------------------------------------
+using ws::bf16;
+
+}  // namespace
+
+struct WsState {
+    ws::Dims d;
+    int ln[4], lk[4];          // layer output / input features
+    int64_t off[4], offw[4];   // offsets: flat FP32 parameters (Wt then b) / BF16 weights
+    int64_t P = 0, PW = 0;
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+    bf16 *wt = nullptr, *wn = nullptr;  // BF16 Wt [N x K] and W = Wt^T [K x N]
+    int64_t adam_t = 0;
+    bool enabled = false;
+    // inference (one row per column)
+    bf16 *X = nullptr, *Ha = nullptr, *Hb = nullptr;
+    float* Y = nullptr;
+    double *gv = nullptr, *ge = nullptr;
+    // training set
+    int64_t rows = 0, cap = 0;
+    bf16* Xd = nullptr;
+    float *Td = nullptr, *Md = nullptr;
+    double *unv = nullptr, *uni = nullptr, *une = nullptr, *u0v = nullptr;
+    // minibatch work
+    int64_t Bm = 0, idx_cap = 0;
+    bf16 *Xb = nullptr, *Hbuf[3] = {nullptr, nullptr, nullptr}, *dY = nullptr, *dZa = nullptr,
+         *dZb = nullptr, *tA = nullptr, *tB = nullptr;
+    float *Tb = nullptr, *Mk = nullptr, *Z[3] = {nullptr, nullptr, nullptr}, *Yb = nullptr,
+          *part = nullptr, *loss = nullptr, *f32tmp = nullptr;
+    int32_t* idx = nullptr;
+};
+
+namespace {
+
+void ws_free_batch(WsState* w) {
+    dfree(w->Xb);
+    for (int l = 0; l < 3; ++l) {
+        dfree(w->Hbuf[l]);
+        dfree(w->Z[l]);
+    }
+    dfree(w->dY);
+    dfree(w->dZa);
+    dfree(w->dZb);
+    dfree(w->tA);
+    dfree(w->tB);
+    dfree(w->Tb);
+    dfree(w->Mk);
+    dfree(w->Yb);
+    dfree(w->part);
+    dfree(w->f32tmp);
+    w->Bm = 0;
+}
+
+void ws_free(WsState* w) {
+    if (!w) return;
+    ws_free_batch(w);
+    dfree(w->p);
+    dfree(w->g);
+    dfree(w->m1);
+    dfree(w->m2);
+    dfree(w->wt);
+    dfree(w->wn);
+    dfree(w->X);
+    dfree(w->Ha);
+    dfree(w->Hb);
+    dfree(w->Y);
+    dfree(w->gv);
+    dfree(w->ge);
+    dfree(w->Xd);
+    dfree(w->Td);
+    dfree(w->Md);
+    dfree(w->unv);
+    dfree(w->uni);
+    dfree(w->une);
+    dfree(w->u0v);
+    dfree(w->loss);
+    dfree(w->idx);
+    delete w;
+}
+
+dim3 grid1(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <int EPI>
+void ws_gemm(clm_ctx* c, int M, int N, int K, const bf16* A, int lda, const bf16* B, int ldb,
+             const float* bias, float* Cf, int ldcf, bf16* Cb, int ldcb, const float* Z, int ldz,
+             int S = 1) {
+    dim3 g((unsigned)((M + ws::kBM - 1) / ws::kBM), (unsigned)((N + ws::kBN - 1) / ws::kBN),
+           (unsigned)S);
+    hipLaunchKernelGGL(ws::k_gemm_nt<EPI>, g, dim3(256), 0, c->stream, M, N, K, K / S, A, lda,
+                       B, ldb, bias, Cf, ldcf, Cb, ldcb, Z, ldz);
+}
+
+// forward pass over M rows of X; train keeps pre-activations and activations
+void ws_forward(clm_ctx* c, WsState* w, const bf16* X, int M, bool train, float* Yout) {
+    const bf16* in = X;
+    int kin = w->d.Fp;
+    const int H = w->d.H;
+    for (int l = 0; l < 3; ++l) {
+        bf16* out = train ? w->Hbuf[l] : (l % 2 == 0 ? w->Ha : w->Hb);
+        ws_gemm<ws::EPI_BIAS_GELU>(c, M, H, kin, in, kin, w->wt + w->offw[l], kin,
+                                   w->p + w->off[l] + (int64_t)H * kin, train ? w->Z[l] : nullptr,
+                                   H, out, H, nullptr, 0);
+        in = out;
+        kin = H;
+    }
+    ws_gemm<ws::EPI_BIAS_F32>(c, M, w->d.Nout, H, in, H, w->wt + w->offw[3], H,
+                              w->p + w->off[3] + (int64_t)w->d.Nout * H, Yout, w->d.Nout, nullptr,
+                              0, nullptr, 0);
+}
+
+// gradient of layer l's weights and bias from its output gradient D [M x N]
+// and its input Hin [M x K]: dWt = D^T Hin, db = column sums of D
+void ws_wgrad(clm_ctx* c, WsState* w, int l, const bf16* D, const bf16* Hin, int M) {
+    const int N = w->ln[l], K = w->lk[l];
+    hipLaunchKernelGGL(ws::k_transpose, dim3((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32)),
+                       dim3(256), 0, c->stream, M, N, D, N, w->tA, M);
+    hipLaunchKernelGGL(ws::k_transpose, dim3((unsigned)((K + 31) / 32), (unsigned)((M + 31) / 32)),
+                       dim3(256), 0, c->stream, M, K, Hin, K, w->tB, M);
+    int S = 1;
+    while (S < 16 && M % (64 * S) == 0 && M / (2 * S) >= 512) S *= 2;
+    ws_gemm<ws::EPI_F32>(c, N, K, M, w->tA, M, w->tB, M, nullptr, w->part, K, nullptr, 0,
+                         nullptr, 0, S);
+    int64_t nk = (int64_t)N * K;
+    hipLaunchKernelGGL(ws::k_sum_slices, grid1(nk), dim3(256), 0, c->stream, nk, S, w->part,
+                       w->g + w->off[l]);
+    hipLaunchKernelGGL(ws::k_colsum, grid1(N), dim3(256), 0, c->stream, M, N, D,
+                       w->g + w->off[l] + nk);
+}
+
+// backward pass from dY (bf16 [M x Nout]) to every parameter gradient
+void ws_backward(clm_ctx* c, WsState* w, int M) {
+    const int H = w->d.H, No = w->d.Nout;
+    ws_wgrad(c, w, 3, w->dY, w->Hbuf[2], M);
+    ws_gemm<ws::EPI_DGELU>(c, M, H, No, w->dY, No, w->wn + w->offw[3], No, nullptr, nullptr, 0,
+                           w->dZa, H, w->Z[2], H);
+    ws_wgrad(c, w, 2, w->dZa, w->Hbuf[1], M);
+    ws_gemm<ws::EPI_DGELU>(c, M, H, H, w->dZa, H, w->wn + w->offw[2], H, nullptr, nullptr, 0,
+                           w->dZb, H, w->Z[1], H);
+    ws_wgrad(c, w, 1, w->dZb, w->Hbuf[0], M);
+    ws_gemm<ws::EPI_DGELU>(c, M, H, H, w->dZb, H, w->wn + w->offw[1], H, nullptr, nullptr, 0,
+                           w->dZa, H, w->Z[0], H);
+    ws_wgrad(c, w, 0, w->dZa, w->Xb, M);
+}
+
+void ws_adam(clm_ctx* c, WsState* w, float lr) {
+    w->adam_t += 1;
+    float b1t = (float)std::pow(0.9, (double)w->adam_t);
+    float b2t = (float)std::pow(0.999, (double)w->adam_t);
+    for (int l = 0; l < 4; ++l) {
+        int N = w->ln[l], K = w->lk[l];
+        int64_t n = (int64_t)N * K + N;
+        float* P = w->p + w->off[l];
+        hipLaunchKernelGGL(ws::k_adam, grid1(n), dim3(256), 0, c->stream, N, K, P,
+                           P + (int64_t)N * K, w->g + w->off[l], w->g + w->off[l] + (int64_t)N * K,
+                           w->m1 + w->off[l], w->m2 + w->off[l], lr, b1t, b2t,
+                           w->wt + w->offw[l], w->wn + w->offw[l]);
+    }
+}
+
+void ws_cast(clm_ctx* c, WsState* w) {
+    for (int l = 0; l < 4; ++l) {
+        int64_t n = (int64_t)w->ln[l] * w->lk[l];
+        hipLaunchKernelGGL(ws::k_cast_weights, grid1(n), dim3(256), 0, c->stream, w->ln[l],
+                           w->lk[l], w->p + w->off[l], w->wt + w->offw[l], w->wn + w->offw[l]);
+    }
+}
+
+int ws_batch_buffers(clm_ctx* c, WsState* w, int64_t M) {
+    if (w->Bm == M) return CLM_OK;
+    ws_free_batch(w);
+    const int64_t H = w->d.H, Fp = w->d.Fp, No = w->d.Nout;
+    int r = 0;
+    const int64_t kmax = std::max(H, Fp), nmax = std::max(H, No);
+    if ((r = dalloc(c, &w->Xb, M * Fp)) || (r = dalloc(c, &w->Tb, M * No)) ||
+        (r = dalloc(c, &w->Mk, M)) || (r = dalloc(c, &w->Yb, M * No)) ||
+        (r = dalloc(c, &w->dY, M * No)) || (r = dalloc(c, &w->dZa, M * H)) ||
+        (r = dalloc(c, &w->dZb, M * H)) || (r = dalloc(c, &w->tA, nmax * M)) ||
+        (r = dalloc(c, &w->tB, kmax * M)) || (r = dalloc(c, &w->part, 16 * nmax * kmax)) ||
+        (r = dalloc(c, &w->f32tmp, M * std::max(Fp, No))))
+        return r;
+    for (int l = 0; l < 3; ++l)
+        if ((r = dalloc(c, &w->Hbuf[l], M * H)) || (r = dalloc(c, &w->Z[l], M * H))) return r;
+    w->Bm = M;
+    return CLM_OK;
+}
+
+int ws_features_launch(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
+                       bf16* X) {
+    hipLaunchKernelGGL(ws::k_features, grid1(c->ncol), dim3(256), 0, c->stream, is_eh(c),
+                       c->d_col_ptr, c->ncol, c->d_cpar, c->d_cv, c->d_ci, c->d_ce, ftw,
+                       fth ? fth : ftw, dt, w->d.Fp, X);
+    return CLM_OK;
+}
+
+void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
+                     StepExtra* x) {
+    WsState* w = c->ws;
+    if (!w || !w->enabled) return;
+    ws_features_launch(c, w, ftw, fth, dt, w->X);
+    ws_forward(c, w, w->X, (int)c->ncol, false, w->Y);
+    hipLaunchKernelGGL(ws::k_guess, grid1(c->ncol), dim3(256), 0, c->stream, is_eh(c),
+                       c->d_col_ptr, c->ncol, c->d_cpar, c->d_cv, c->d_ci, w->Y, w->d.Nout,
+                       w->gv, w->ge);
+    x->gv = w->gv;
+    x->ge = is_eh(c) ? w->ge : nullptr;
 }
 
 int upload_raw(clm_ctx* c, const double* h, double* d, int64_t n) {
@@ -789,6 +1013,8 @@ int clm_destroy(clm_ctx* c) {
     dfree(c->d_cfbw);
     dfree(c->d_cfbh);
     for (auto& bn : c->bins) dfree(bn.d_cols);
+    ws_free(c->ws);
+    c->ws = nullptr;
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
@@ -1358,6 +1584,301 @@ int clm_last_kernel_ms(const clm_ctx* c, double* ms_per_launch, int32_t* launche
     if (!c || !ms_per_launch) return CLM_E_ARG;
     *ms_per_launch = c->last_launches ? c->last_ms / c->last_launches : 0.0;
     if (launches) *launches = c->last_launches;
+    return CLM_OK;
+}
+
+
+// ---- learned warm start ---------------------------------------------------------
+
+int clm_ws_init(clm_ctx* c, uint64_t seed) {
+    if (!c) return CLM_E_ARG;
+    if (!is_group(c)) return set_err(c, CLM_E_STATE, "the warm start needs the group solver");
+    ws_free(c->ws);
+    c->ws = nullptr;
+    destroy_graph(c);
+    WsState* w = new WsState();
+    w->d = ws::dims_for(is_eh(c));
+    const int H = w->d.H;
+    const int ln[4] = {H, H, H, w->d.Nout}, lk[4] = {w->d.Fp, H, H, H};
+    for (int l = 0; l < 4; ++l) {
+        w->ln[l] = ln[l];
+        w->lk[l] = lk[l];
+        w->off[l] = w->P;
+        w->offw[l] = w->PW;
+        w->P += (int64_t)ln[l] * lk[l] + ln[l];
+        w->PW += (int64_t)ln[l] * lk[l];
+    }
+    int r = 0;
+    if ((r = dalloc(c, &w->p, w->P)) || (r = dalloc(c, &w->g, w->P)) ||
+        (r = dalloc(c, &w->m1, w->P)) || (r = dalloc(c, &w->m2, w->P)) ||
+        (r = dalloc(c, &w->wt, w->PW)) || (r = dalloc(c, &w->wn, w->PW)) ||
+        (r = dalloc(c, &w->X, c->ncol * w->d.Fp)) || (r = dalloc(c, &w->Ha, c->ncol * H)) ||
+        (r = dalloc(c, &w->Hb, c->ncol * H)) || (r = dalloc(c, &w->Y, c->ncol * w->d.Nout)) ||
+        (r = dalloc(c, &w->gv, c->ncell)) || (r = dalloc(c, &w->ge, c->ncell)) ||
+        (r = dalloc(c, &w->loss, 1))) {
+        ws_free(w);
+        return r;
+    }
+    // uniform(-a, a) weights, a = sqrt(6 / fan_in) (output layer x 0.1); zero biases
+    std::vector<float> h((size_t)w->P, 0.0f);
+    uint64_t st = seed ^ 0x5DEECE66DULL;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    for (int l = 0; l < 4; ++l) {
+        double a = std::sqrt(6.0 / lk[l]) * (l == 3 ? 0.1 : 1.0);
+        for (int64_t i = 0; i < (int64_t)ln[l] * lk[l]; ++i) {
+            double u = (double)(next() >> 11) * (1.0 / 9007199254740992.0);
+            h[(size_t)(w->off[l] + i)] = (float)((2.0 * u - 1.0) * a);
+        }
+    }
+    hipError_t e = hipMemcpyAsync(w->p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        ws_cast(c, w);
+        e = hipStreamSynchronize(c->stream);
+    }
+    if (e != hipSuccess) {
+        ws_free(w);
+        return set_err(c, CLM_E_HIP, "warm-start init failed: %s", hipGetErrorString(e));
+    }
+    c->ws = w;
+    return CLM_OK;
+}
+
+int clm_ws_dims(clm_ctx* c, int32_t* dims, int64_t* nparams) {
+    if (!c || !dims) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    dims[0] = c->ws->d.F;
+    dims[1] = c->ws->d.Fp;
+    dims[2] = c->ws->d.H;
+    dims[3] = c->ws->d.Nout;
+    if (nparams) *nparams = c->ws->P;
+    return CLM_OK;
+}
+
+int clm_ws_get_params(clm_ctx* c, float* out) {
+    if (!c || !out) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    HIPCHK(c, hipMemcpyAsync(out, c->ws->p, 4 * (size_t)c->ws->P, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int clm_ws_set_params(clm_ctx* c, const float* in) {
+    if (!c || !in) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    HIPCHK(c, hipMemcpyAsync(c->ws->p, in, 4 * (size_t)c->ws->P, hipMemcpyHostToDevice,
+                             c->stream));
+    ws_cast(c, c->ws);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int clm_ws_enable(clm_ctx* c, int32_t on) {
+    if (!c) return CLM_E_ARG;
+    if (on && !c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    if (c->ws && c->ws->enabled != (on != 0)) destroy_graph(c);
+    if (c->ws) c->ws->enabled = on != 0;
+    return CLM_OK;
+}
+
+int clm_ws_features(clm_ctx* c, double dt, float* out) {
+    if (!c || !out) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    WsState* w = c->ws;
+    ws_features_launch(c, w, c->d_cftw, is_eh(c) ? c->d_cfth : nullptr, dt, w->X);
+    int64_t n = c->ncol * w->d.Fp;
+    float* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, 4 * (size_t)n));
+    hipLaunchKernelGGL(ws::k_bf16_to_f32, grid1(n), dim3(256), 0, c->stream, n, w->X, tmp);
+    hipError_t e = hipMemcpyAsync(out, tmp, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return set_err(c, CLM_E_HIP, "features: %s", hipGetErrorString(e));
+    return CLM_OK;
+}
+
+int clm_ws_collect(clm_ctx* c, double dt, int32_t nsteps, int32_t steps_per_slot,
+                   int64_t* rows_out) {
+    if (!c || nsteps < 1 || steps_per_slot < 1 || !(dt > 0.0)) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    if (c->tab_slots < 1) return set_err(c, CLM_E_STATE, "clm_set_forcing_table must run first");
+    WsState* w = c->ws;
+    const int Fp = w->d.Fp, No = w->d.Nout;
+    int64_t need = w->rows + (int64_t)nsteps * c->ncol;
+    int r = 0;
+    if (need > w->cap) {  // grow the training set, keeping its rows
+        bf16* X2 = nullptr;
+        float *T2 = nullptr, *M2 = nullptr;
+        if ((r = dalloc(c, &X2, need * Fp)) || (r = dalloc(c, &T2, need * No)) ||
+            (r = dalloc(c, &M2, need)))
+            return r;
+        if (w->rows) {
+            HIPCHK(c, hipMemcpyAsync(X2, w->Xd, 2 * (size_t)(w->rows * Fp), hipMemcpyDeviceToDevice,
+                                     c->stream));
+            HIPCHK(c, hipMemcpyAsync(T2, w->Td, 4 * (size_t)(w->rows * No), hipMemcpyDeviceToDevice,
+                                     c->stream));
+            HIPCHK(c, hipMemcpyAsync(M2, w->Md, 4 * (size_t)w->rows, hipMemcpyDeviceToDevice,
+                                     c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        dfree(w->Xd);
+        dfree(w->Td);
+        dfree(w->Md);
+        w->Xd = X2;
+        w->Td = T2;
+        w->Md = M2;
+        w->cap = need;
+    }
+    if (!w->u0v) {
+        if ((r = dalloc(c, &w->u0v, c->ncell)) || (r = dalloc(c, &w->unv, c->ncell)) ||
+            (r = dalloc(c, &w->uni, c->ncell)) || (r = dalloc(c, &w->une, c->ncell)))
+            return r;
+    }
+    if ((r = ensure_iters(c, 1))) return r;
+    const bool eh = is_eh(c);
+    for (int32_t s = 0; s < nsteps; ++s) {
+        int64_t slot = (s / steps_per_slot) % c->tab_slots;
+        const double* ftw = c->d_tab_w + slot * c->ncol;
+        const double* fth = eh ? c->d_tab_h + slot * c->ncol : nullptr;
+        ws_features_launch(c, w, ftw, fth, dt, w->Xd + w->rows * Fp);
+        HIPCHK(c, hipMemcpyAsync(w->unv, c->d_cv, 8 * (size_t)c->ncell, hipMemcpyDeviceToDevice,
+                                 c->stream));
+        if (eh) {
+            HIPCHK(c, hipMemcpyAsync(w->uni, c->d_ci, 8 * (size_t)c->ncell,
+                                     hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(w->une, c->d_ce, 8 * (size_t)c->ncell,
+                                     hipMemcpyDeviceToDevice, c->stream));
+        }
+        launch_step(c, ftw, fth, dt, c->d_iters, w->u0v, false);
+        hipLaunchKernelGGL(ws::k_targets, grid1(c->ncol), dim3(256), 0, c->stream, eh,
+                           c->d_col_ptr, c->ncol, c->d_cpar, w->u0v, w->unv, w->uni, w->une,
+                           c->d_cv, c->d_ce, c->d_iters, No, w->Td + w->rows * No,
+                           w->Md + w->rows);
+        HIPCHK(c, hipGetLastError());
+        w->rows += c->ncol;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_cache = false;
+    c->have_jac = false;
+    if (rows_out) *rows_out = w->rows;
+    return CLM_OK;
+}
+
+int clm_ws_train(clm_ctx* c, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                 float* loss_per_epoch) {
+    if (!c || epochs < 1 || batch < 512 || batch % 512 != 0 || !(lr > 0.0f)) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    WsState* w = c->ws;
+    int64_t nb = w->rows / batch;
+    if (nb < 1) return set_err(c, CLM_E_STATE, "training set holds fewer rows than one batch");
+    int r = ws_batch_buffers(c, w, batch);
+    if (r) return r;
+    if (w->idx_cap < w->rows) {
+        dfree(w->idx);
+        if ((r = dalloc(c, &w->idx, w->rows))) return r;
+        w->idx_cap = w->rows;
+    }
+    std::vector<int32_t> perm((size_t)w->rows);
+    std::iota(perm.begin(), perm.end(), 0);
+    uint64_t st = seed;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    const int Fp = w->d.Fp, No = w->d.Nout;
+    for (int32_t ep = 0; ep < epochs; ++ep) {
+        for (int64_t i = w->rows - 1; i > 0; --i) std::swap(perm[i], perm[next() % (i + 1)]);
+        HIPCHK(c, hipMemcpyAsync(w->idx, perm.data(), 4 * perm.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        hipLaunchKernelGGL(ws::k_zero_f32, dim3(1), dim3(64), 0, c->stream, (int64_t)1, w->loss);
+        for (int64_t b = 0; b < nb; ++b) {
+            hipLaunchKernelGGL(ws::k_gather, dim3((unsigned)batch), dim3(128), 0, c->stream, batch,
+                               w->idx + b * batch, Fp, w->Xd, No, w->Td, w->Md, w->Xb, w->Tb,
+                               w->Mk);
+            ws_forward(c, w, w->Xb, batch, true, w->Yb);
+            hipLaunchKernelGGL(ws::k_mse, grid1((int64_t)batch * No), dim3(256), 0, c->stream,
+                               batch, No, w->Yb, w->Tb, w->Mk, w->dY, w->loss);
+            ws_backward(c, w, batch);
+            ws_adam(c, w, lr);
+        }
+        HIPCHK(c, hipGetLastError());
+        float l = 0.0f;
+        HIPCHK(c, hipMemcpyAsync(&l, w->loss, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (loss_per_epoch) loss_per_epoch[ep] = l / (float)nb;
+    }
+    return CLM_OK;
+}
+
+int clm_ws_forward(clm_ctx* c, const float* X, int64_t M, float* Y) {
+    if (!c || !X || !Y || M < 1 || M % 32 != 0) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    WsState* w = c->ws;
+    int r = ws_batch_buffers(c, w, M);
+    if (r) return r;
+    const int64_t nx = M * w->d.Fp, ny = M * w->d.Nout;
+    HIPCHK(c, hipMemcpyAsync(w->f32tmp, X, 4 * (size_t)nx, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(ws::k_f32_to_bf16, grid1(nx), dim3(256), 0, c->stream, nx, w->f32tmp,
+                       w->Xb);
+    ws_forward(c, w, w->Xb, (int)M, true, w->Yb);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(Y, w->Yb, 4 * (size_t)ny, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+int clm_ws_gradient(clm_ctx* c, const float* X, const float* T, const float* mask, int64_t M,
+                    float* loss, float* grad) {
+    if (!c || !X || !T || !mask || !grad || M < 32 || M % 32 != 0) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    WsState* w = c->ws;
+    int r = ws_batch_buffers(c, w, M);
+    if (r) return r;
+    const int64_t nx = M * w->d.Fp, ny = M * w->d.Nout;
+    HIPCHK(c, hipMemcpyAsync(w->f32tmp, X, 4 * (size_t)nx, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(ws::k_f32_to_bf16, grid1(nx), dim3(256), 0, c->stream, nx, w->f32tmp,
+                       w->Xb);
+    HIPCHK(c, hipMemcpyAsync(w->Tb, T, 4 * (size_t)ny, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(w->Mk, mask, 4 * (size_t)M, hipMemcpyHostToDevice, c->stream));
+    ws_forward(c, w, w->Xb, (int)M, true, w->Yb);
+    hipLaunchKernelGGL(ws::k_zero_f32, dim3(1), dim3(64), 0, c->stream, (int64_t)1, w->loss);
+    hipLaunchKernelGGL(ws::k_mse, grid1(ny), dim3(256), 0, c->stream, (int)M, w->d.Nout, w->Yb,
+                       w->Tb, w->Mk, w->dY, w->loss);
+    ws_backward(c, w, (int)M);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(grad, w->g, 4 * (size_t)w->P, hipMemcpyDeviceToHost, c->stream));
+    if (loss) HIPCHK(c, hipMemcpyAsync(loss, w->loss, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return CLM_OK;
+}
+
+
+int clm_ws_time(clm_ctx* c, double dt, int32_t reps, double* ms_per_call) {
+    if (!c || reps < 1 || !ms_per_call || !(dt > 0.0)) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    StepExtra x;
+    bool was = c->ws->enabled;
+    c->ws->enabled = true;
+    ws_launch_guess(c, c->d_cftw, c->d_cfth, dt, &x);  // warm-up
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int32_t i = 0; i < reps; ++i) ws_launch_guess(c, c->d_cftw, c->d_cfth, dt, &x);
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->ws->enabled = was;
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    *ms_per_call = ms / reps;
     return CLM_OK;
 }
 

@@ -222,6 +222,45 @@ int clm_autotune(clm_ctx* ctx, double dt, int32_t reps);
  * launch (ms), measured with HIP events on the ctx stream. */
 int clm_last_kernel_ms(const clm_ctx* ctx, double* ms_per_launch, int32_t* launches);
 
+/* ---- learned Newton warm start (group solver) ---------------------------------
+ * A per-column MLP predicts the implicit part of the step's increment,
+ * x_{n+1} - U0, at 32 nodes (layer-index resampling); Newton then starts at
+ * x0 = U0 + guess and iterates to the configured tolerance, falling back to
+ * x0 = U0 when the warm-started iterate turns non-finite (flag bit 0x80000;
+ * iteration counts then include both attempts).  The network is the
+ * project's own: BF16 weights/activations with FP32 accumulation on MFMA,
+ * GELU, layers F -> H -> H -> H -> Nout with (F, H, Nout) = (105, 384, 64)
+ * for EnergyHydrology (inputs padded to 128) and (40, 256, 32) for
+ * RichardsModel (padded to 64).  No counterpart in the reference: it replaces
+ * Newton's initial guess x0 = U0 (the predictor of ARS111 in
+ * ClimaTimeSteppers' NewtonsMethod) and nothing else.
+ * Parameters are one FP32 vector: per layer Wt [N x K] row-major, then b [N]. */
+int clm_ws_init(clm_ctx* ctx, uint64_t seed);            /* allocate + random init */
+int clm_ws_dims(clm_ctx* ctx, int32_t* dims /* F, Fp, H, Nout */, int64_t* nparams);
+int clm_ws_get_params(clm_ctx* ctx, float* params);
+int clm_ws_set_params(clm_ctx* ctx, const float* params);
+/* Use (1) or stop using (0) the warm start in clm_step / clm_run. */
+int clm_ws_enable(clm_ctx* ctx, int32_t on);
+/* Training pairs: advance the device state nsteps plain steps with the
+ * clm_set_forcing_table slots and append (features of u_n, node targets of
+ * x_{n+1} - U0, converged mask) per column-step; returns the row count. */
+int clm_ws_collect(clm_ctx* ctx, double dt, int32_t nsteps, int32_t steps_per_slot,
+                   int64_t* rows);
+/* Adam on the collected rows (masked MSE), minibatch `batch` (multiple of
+ * 512), one loss value per epoch. */
+int clm_ws_train(clm_ctx* ctx, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                 float* loss_per_epoch);
+/* Checking hooks: features of the device state ([ncol x Fp], with the
+ * clm_set_forcing fluxes), a forward pass of host rows X [M x Fp] -> Y
+ * [M x Nout], and the loss and parameter gradient for host rows (M % 32 == 0). */
+int clm_ws_features(clm_ctx* ctx, double dt, float* X);
+int clm_ws_forward(clm_ctx* ctx, const float* X, int64_t M, float* Y);
+int clm_ws_gradient(clm_ctx* ctx, const float* X, const float* T, const float* mask, int64_t M,
+                    float* loss, float* grad);
+/* Device time of one warm-start inference (features, network, guess) on the
+ * current state and clm_set_forcing fluxes, averaged over reps (HIP events). */
+int clm_ws_time(clm_ctx* ctx, double dt, int32_t reps, double* ms_per_call);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,207 @@
+"""GPU checks of the learned Newton warm start (clm_ws_* through the C ABI).
+
+Bars:
+  * features: equal to the CPU restatement rounded to BF16 on >= 99.5% of
+    entries, every entry within 1 BF16 ulp (2^-7 relative) or 1e-6;
+  * network forward / loss / gradient vs the CPU restatement with the same
+    BF16 rounding points: relative L2 error <= 1e-2 per output and per
+    layer gradient (rounding of intermediates may flip on near-ties);
+  * warm start with a zero guess: the oracle's warm-started step (x0 = U0)
+    to 1e-10 on well-conditioned columns;
+  * trained warm start: same root as plain Newton to the Newton tolerance,
+    and it reports the iterations it saves.
+"""
+import numpy as np
+import pytest
+
+from conftest import column_ok, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(batch, jac=64):
+    from clmsoil.model import model_from_batch
+    return model_from_batch(batch, jac_precision=jac, solver=1)
+
+
+def _set_forcing(m, b, slot=0):
+    from clmsoil.model import HeatFluxBC, WaterFluxBC, WaterHeatBC
+    if b.top_heat is None:
+        m.boundary_conditions["top"] = WaterFluxBC(b.top_water[slot])
+    else:
+        m.boundary_conditions["top"] = WaterHeatBC(WaterFluxBC(b.top_water[slot]),
+                                                   HeatFluxBC(b.top_heat[slot]))
+    m.set_forcing()
+
+
+@pytest.mark.parametrize("name", ["rre_131072_uneven", "eh_262144_uneven"])
+def test_features_match_cpu(gpu_lib, name):
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart
+    from oracle import mlp_oracle as mo
+    from oracle.oracle import constants
+    b = inputs.make_batch(name, ncol=300)
+    m = _model(b)
+    _set_forcing(m, b)
+    sim = LandSimulation(m, b.copy_state(), b.workload.dt)
+    sim._push()
+    ws = NewtonWarmStart(m, seed=1)
+    X = ws.features(b.workload.dt)
+    ref = mo.features(b, b.copy_state(), b.top_water[0], None if b.top_heat is None else
+                      b.top_heat[0], b.workload.dt, constants())
+    refb = mo.to_bf16(ref.astype(np.float32))
+    same = X == refb
+    print(f"{name}: features equal on {same.mean():.5f} of entries", flush=True)
+    assert same.mean() >= 0.995
+    assert (np.abs(X - refb) <= np.maximum(2.0 ** -7 * np.abs(refb), 1e-6)).all()
+    m.close()
+
+
+@pytest.mark.parametrize("eh", [False, True])
+def test_network_forward_and_gradient_match_cpu(gpu_lib, eh):
+    from clmsoil import inputs
+    from clmsoil.model import NewtonWarmStart
+    from oracle import mlp_oracle as mo
+    b = inputs.make_batch("eh_262144_uneven" if eh else "rre_131072_uneven", ncol=64)
+    m = _model(b)
+    ws = NewtonWarmStart(m, seed=7)
+    F, Fp, H, No = mo.dims(eh)
+    assert (ws.F, ws.Fp, ws.H, ws.Nout) == (F, Fp, H, No)
+    rng = np.random.default_rng(11)
+    M = 256
+    X = np.zeros((M, Fp), dtype=np.float32)
+    X[:, :F] = rng.standard_normal((M, F))
+    T = rng.standard_normal((M, No)).astype(np.float32)
+    mask = (rng.random(M) > 0.25).astype(np.float32)
+    p = ws.params()
+    # perturb the output layer so that every layer carries a sizeable gradient
+    p = (p + 0.02 * rng.standard_normal(p.size)).astype(np.float32)
+    ws.set_params(p)
+    Y = ws.forward(X)
+    Yr, _ = mo.forward(p, eh, X)
+    ey = np.linalg.norm(Y - Yr) / np.linalg.norm(Yr)
+    loss, g = ws.gradient(X, T, mask)
+    lr, gr = mo.loss_and_grad(p, eh, X, T, mask.astype(np.float64))
+    print(f"eh={eh}: forward rel err {ey:.2e}; loss {loss:.6g} vs {lr:.6g}", flush=True)
+    assert ey <= 1e-2
+    assert abs(loss - lr) <= 1e-2 * abs(lr)
+    o = 0
+    for li, (n, k) in enumerate(mo.layer_shapes(eh)):
+        for part, size in (("W", n * k), ("b", n)):
+            a, r = g[o:o + size], gr[o:o + size]
+            e = np.linalg.norm(a - r) / np.linalg.norm(r)
+            print(f"  layer {li} {part}: grad rel err {e:.2e} (|g| {np.linalg.norm(r):.3g})",
+                  flush=True)
+            assert e <= 1e-2, (li, part, e)
+            o += size
+    assert o == p.size
+    m.close()
+
+
+def _one_step(b, sim, m):
+    w = b.workload
+    th = None if b.top_heat is None else b.top_heat[:1]
+    _, it = sim.solve(b.top_water[:1], th, 1, 1, want_iters=True)
+    return sim.state(), it[0]
+
+
+@pytest.mark.parametrize("name", ["rre_131072_uneven", "eh_262144_uneven"])
+def test_zero_guess_matches_oracle_warm_start(gpu_lib, name):
+    """With all network parameters zero the guess is exactly 0: the fused
+    kernels' warm-start path must equal the oracle's orc_step_guess."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart
+    from oracle.oracle import OracleModel
+    b = inputs.make_batch(name, ncol=512)
+    w = b.workload
+    m = _model(b)
+    ws = NewtonWarmStart(m, seed=3)
+    ws.set_params(np.zeros(ws.n_params, dtype=np.float32))
+    ws.enable(True)
+    sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters, tol=w.newton_tol)
+    Yg, itg = _one_step(b, sim, m)
+    fw = b.top_water[0].copy()
+    fh = None if b.top_heat is None else b.top_heat[0].copy()
+    zero = np.zeros(b.ncell)
+    outs = {}
+    for v in ("", "xprec", "fma"):
+        o = OracleModel.from_batch(b, variant=v)
+        Y = b.copy_state()
+        it, _ = o.step(Y, w.dt, w.newton_max_iters, w.newton_tol, fw, fh, guess_v=zero,
+                       guess_e=zero if b.top_heat is not None else None)
+        outs[v] = (Y, it)
+    Yo, ito = outs[""]
+    ok = column_ok(b, Yo) & column_ok(b, Yg) & (ito < w.newton_max_iters)
+    sens = np.zeros(b.ncol)
+    keys = ["vartheta_l"] + (["rho_e_int"] if b.top_heat is not None else [])
+    for v in ("xprec", "fma"):
+        Yv, itv = outs[v]
+        ok &= column_ok(b, Yv) & (itv == ito)
+        for k in keys:
+            fl = 1.0 if k == "rho_e_int" else 1e-12
+            sens = np.maximum(sens, np.maximum.reduceat(
+                np.where(np.repeat(ok, b.nlayers), rel_err(Yv[k], Yo[k], fl), 0.0),
+                b.col_ptr[:-1]))
+    well = ok & (sens <= 1e-12) & ((itg & 0xFFFF) == ito)
+    assert well.mean() > 0.6, well.mean()
+    assert ((itg & 0xFFFF) == ito)[ok].mean() >= 0.98
+    assert not (itg & 0x80000).any()
+    for k in keys:
+        fl = 1.0 if k == "rho_e_int" else 1e-12
+        e = np.where(np.repeat(well, b.nlayers), rel_err(Yg[k], Yo[k], fl), 0.0)
+        print(f"{name} {k}: zero-guess warm start max rel err {e.max():.2e} on {well.sum()} "
+              f"columns", flush=True)
+        assert e.max() <= 1e-10
+    m.close()
+
+
+@pytest.mark.parametrize("name,jac", [("eh_262144_uneven", 32), ("rre_131072_uneven", 64)])
+def test_trained_warm_start_saves_iterations(gpu_lib, name, jac):
+    """Train on columns from other seeds, evaluate on the workload's own
+    columns: same converged states as plain Newton, fewer iterations."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart
+    tname = inputs.training_workload(name)
+    bt = inputs.make_batch(tname, ncol=8192, nsteps=12)
+    w = bt.workload
+    m = _model(bt, jac)
+    ws = NewtonWarmStart(m, seed=5)
+    sim = LandSimulation(m, bt.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                         tol=w.newton_tol)
+    rows = ws.collect(sim, bt.top_water, bt.top_heat, 12, w.steps_per_slot)
+    assert rows == 12 * 8192
+    losses = ws.train(epochs=8, batch=2048, lr=1e-3, seed=9)
+    print(f"{name}: training loss by epoch {np.array2string(losses, precision=4)}", flush=True)
+    assert np.isfinite(losses).all() and losses[-1] < 0.7 * losses[0]
+    params = ws.params()
+    m.close()
+    # evaluation on the workload's own first columns, steps 0..5
+    b = inputs.make_batch(name, ncol=4096, nsteps=6)
+    res = {}
+    for on in (False, True):
+        m2 = _model(b, jac)
+        ws2 = NewtonWarmStart(m2, seed=0)
+        ws2.set_params(params)
+        ws2.enable(on)
+        s2 = LandSimulation(m2, b.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                            tol=w.newton_tol)
+        st, it = s2.solve(b.top_water, b.top_heat, 6, w.steps_per_slot, want_iters=True)
+        res[on] = (s2.state(), it)
+        m2.close()
+    (Yp, itp), (Yw, itw) = res[False], res[True]
+    good = ~(((itp | itw) & 0x70000) != 0).any(axis=0)
+    good &= column_ok(b, Yp) & column_ok(b, Yw)
+    kp, kw = (itp & 0xFFFF)[:, good], (itw & 0xFFFF)[:, good]
+    fb = ((itw & 0x80000) != 0).mean()
+    dcol = np.maximum.reduceat(np.abs(Yw["vartheta_l"] - Yp["vartheta_l"]), b.col_ptr[:-1])[good]
+    q50, q99 = np.quantile(dcol, [0.5, 0.99])
+    print(f"{name}: mean Newton iterations plain {kp.mean():.3f} -> warm start {kw.mean():.3f} "
+          f"({100 * (1 - kw.mean() / kp.mean()):.1f}% fewer) on {good.sum()} columns x 6 steps; "
+          f"fallbacks {fb:.4f}; per-column max |Δϑ_l| vs plain: median {q50:.1e}, 99% {q99:.1e}, "
+          f"max {dcol.max():.1e}", flush=True)
+    assert good.mean() > 0.5
+    # two Newton solutions to 1e-8 from different starts agree to about the
+    # tolerance; over six steps a few slowly contracting (modified-Picard)
+    # columns separate further, as they do under a 1-ulp change of the input
+    assert q50 <= 1e-7 and q99 <= 1e-5
+    assert kw.mean() <= kp.mean()
