@@ -12,8 +12,8 @@ Forms: "learned" = the learned Newton warm start on top of "mixed";
 "fp64" = the plain FP64 path.  `value` is the chosen form's throughput; the
 plain FP64 throughput on the same workload is reported beside it.  The
 learned form's network is trained first (untimed) on columns drawn from other
-seeds of the same workload: 8 plain steps of all its columns give the
-(state, converged step) pairs.
+seeds of the same workload: 8 plain steps of all its columns, spread evenly
+over the timed horizon, give the (state, converged step) pairs.
 """
 import argparse
 import json
@@ -87,27 +87,31 @@ def parse_shapes(spec):
     return out
 
 
-def train_warm_start(name, ncol, jac, epochs, collect_steps=8, batch=8192):
+def train_warm_start(name, ncol, jac, epochs, horizon, collect=8, batch=8192):
     """Collect (state, converged step) pairs with the plain solver on columns
-    of other seeds and train the warm-start network on the device."""
+    of other seeds -- `collect` steps spread evenly over the run's `horizon`
+    -- and train the warm-start network on the device."""
     import torch
     from clmsoil import inputs
     from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
-    bt = inputs.make_batch(inputs.training_workload(name), ncol=ncol, nsteps=collect_steps)
+    bt = inputs.make_batch(inputs.training_workload(name), ncol=ncol, nsteps=horizon)
     w = bt.workload
     m = model_from_batch(bt, jac_precision=jac, solver=1)
     ws = NewtonWarmStart(m, seed=5)
     sim = LandSimulation(m, bt.copy_state(), w.dt, max_iters=w.newton_max_iters,
                          tol=w.newton_tol)
     t0 = time.perf_counter()
-    rows = ws.collect(sim, bt.top_water, bt.top_heat, collect_steps, w.steps_per_slot)
+    stride = max(1, horizon // collect)
+    rows = ws.collect(sim, bt.top_water, bt.top_heat, stride * collect, w.steps_per_slot,
+                      stride)
     t1 = time.perf_counter()
     losses = ws.train(epochs, batch=batch, lr=1e-3, seed=9)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     params = ws.params()
     m.close()
-    info = {"pairs": rows, "epochs": epochs, "batch": batch, "params": int(params.size),
+    info = {"pairs": rows, "collected_every": stride, "epochs": epochs, "batch": batch,
+            "params": int(params.size),
             "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
             "collect_s": t1 - t0, "train_s": t2 - t1}
     print(f"[bench] warm start: {rows} pairs collected in {t1 - t0:.1f} s, {epochs} epochs in "
@@ -184,7 +188,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="eh_262144_uneven")
     ap.add_argument("--form", default="learned", choices=["learned", "mixed", "fp64"])
-    ap.add_argument("--epochs", type=int, default=8, help="warm-start training epochs")
+    ap.add_argument("--epochs", type=int, default=16, help="warm-start training epochs")
     ap.add_argument("--ncol", type=int, default=None, help="prefix of the workload (testing)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -219,7 +223,7 @@ def main():
     res = {}
     ws_params, ws_info = None, None
     if args.form == "learned":
-        ws_params, ws_info = train_warm_start(w.name, batch.ncol, chosen, args.epochs)
+        ws_params, ws_info = train_warm_start(w.name, batch.ncol, chosen, args.epochs, steps)
     for name, jac in (("fp64", 64), ("mixed", 32), ("learned", forms["learned"])):
         if name != args.form and (args.only or (batch.model == 0 and name == "mixed")):
             continue

@@ -115,8 +115,8 @@ def load(path: str | None = None) -> C.CDLL:
         "clm_ws_get_params": ([vp, _fp], C.c_int),
         "clm_ws_set_params": ([vp, _fp], C.c_int),
         "clm_ws_enable": ([vp, C.c_int32], C.c_int),
-        "clm_ws_collect": ([vp, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int64)],
-                           C.c_int),
+        "clm_ws_collect": ([vp, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                            C.POINTER(C.c_int64)], C.c_int),
         "clm_ws_train": ([vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp], C.c_int),
         "clm_ws_features": ([vp, C.c_double, _fp], C.c_int),
         "clm_ws_forward": ([vp, _fp, C.c_int64, _fp], C.c_int),

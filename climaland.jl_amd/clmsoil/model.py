@@ -360,13 +360,15 @@ class NewtonWarmStart:
         self.model._check(self.model._lib.clm_ws_enable(self.model._ctx, 1 if on else 0))
 
     def collect(self, sim: "LandSimulation", top_water, top_heat, nsteps: int,
-                steps_per_slot: int) -> int:
-        """Advance `sim` nsteps plain steps on the device, keeping the pairs."""
+                steps_per_slot: int, stride: int = 1) -> int:
+        """Advance `sim` nsteps plain steps on the device, keeping the pairs
+        of every stride-th step."""
         sim.set_forcing_table(top_water, top_heat)
         sim._push()
         rows = C.c_int64(0)
         m = self.model
-        m._check(m._lib.clm_ws_collect(m._ctx, sim.Δt, nsteps, steps_per_slot, C.byref(rows)))
+        m._check(m._lib.clm_ws_collect(m._ctx, sim.Δt, nsteps, steps_per_slot, stride,
+                                       C.byref(rows)))
         sim.t += nsteps * sim.Δt
         return int(rows.value)
 

@@ -18,6 +18,8 @@
 // EnergyHydrology the ρe-block unknown y_ρ = Δρe / ρc_s is exactly the
 // scaled quantity of the convergence norm.
 #pragma once
+#include <type_traits>
+
 #include "clm_eh.h"
 
 namespace clm {
@@ -85,6 +87,78 @@ __device__ __forceinline__ void pcr2(int gl, const JT* a, const JT* b, const JT*
     x[1] = xo;
     x[0] = (d[0] - a[0] * xm - c[0] * xo) / b[0];
 }
+
+// Factor once, solve many: the multipliers of pcr1's elimination depend on
+// the matrix only, so a matrix that stays fixed across Newton iterations (the
+// EnergyHydrology ρe block: lagged κ, lagged ρc_s) is reduced once per step
+// and each iteration only carries its right-hand side through the levels.
+template <int G>
+struct Log2 {
+    static constexpr int v = G <= 1 ? 0 : 1 + Log2<G / 2>::v;
+};
+
+template <int G, typename JT>
+struct Pcr1Factor {
+    static constexpr int L = Log2<G>::v;
+    JT al[L], ga[L];
+    JT ib;
+    __device__ __forceinline__ void factor(int gl, JT a, JT b, JT c) {
+        int lev = 0;
+#pragma unroll
+        for (int s = 1; s < G; s <<= 1, ++lev) {
+            JT ibv = (JT)1 / b;
+            JT am = g_up<G>(a, s), cm = g_up<G>(c, s), ibm = g_up<G>(ibv, s);
+            JT ap = g_down<G>(a, s), cp = g_down<G>(c, s), ibp = g_down<G>(ibv, s);
+            if (gl < s) { am = 0; cm = 0; ibm = 1; }
+            if (gl + s >= G) { ap = 0; cp = 0; ibp = 1; }
+            JT l = -a * ibm, g = -c * ibp;
+            al[lev] = l;
+            ga[lev] = g;
+            b = b + l * cm + g * ap;
+            a = l * am;
+            c = g * cp;
+        }
+        ib = (JT)1 / b;
+    }
+    __device__ __forceinline__ JT solve(int gl, JT d) const {
+        int lev = 0;
+#pragma unroll
+        for (int s = 1; s < G; s <<= 1, ++lev) {
+            JT dm = g_up<G>(d, s), dp = g_down<G>(d, s);
+            if (gl < s) dm = 0;
+            if (gl + s >= G) dp = 0;
+            d = d + al[lev] * dm + ga[lev] * dp;
+        }
+        return d * ib;
+    }
+};
+
+// pcr2 factored: the even-row elimination multipliers, the reduced odd
+// system's factor and what the even-row recovery needs
+template <int G, typename JT>
+struct Pcr2Factor {
+    JT al, ga, a0, c0, ib0;
+    Pcr1Factor<G, JT> odd;
+    __device__ __forceinline__ void factor(int gl, const JT* a, const JT* b, const JT* c) {
+        JT a2 = g_down<G>(a[0], 1), b2 = g_down<G>(b[0], 1), c2 = g_down<G>(c[0], 1);
+        if (gl + 1 >= G) { a2 = 0; b2 = 1; c2 = 0; }
+        ib0 = (JT)1 / b[0];
+        al = -a[1] * ib0;
+        ga = -c[1] / b2;
+        a0 = a[0];
+        c0 = c[0];
+        odd.factor(gl, al * a[0], b[1] + al * c[0] + ga * a2, ga * c2);
+    }
+    __device__ __forceinline__ void solve(int gl, const JT* d, JT* x) const {
+        JT d2 = g_down<G>(d[0], 1);
+        if (gl + 1 >= G) d2 = 0;
+        JT xo = odd.solve(gl, d[1] + al * d[0] + ga * d2);
+        JT xm = g_up<G>(xo, 1);
+        if (gl == 0) xm = 0;
+        x[1] = xo;
+        x[0] = (d[0] - a0 * xm - c0 * xo) * ib0;
+    }
+};
 
 // R >= 2 rows per lane: the partition method.  The lane's interior rows
 // 1..R-2 form a tridiagonal system with its first and last unknowns as
@@ -380,6 +454,32 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         wk[k] = (r == 0) ? 0.0 : 0.5 * (km + kl[k]) / dzf[k];
     }
 
+    // the ρe block of W is fixed for the whole step (lagged κ and ρc_s): with
+    // one or two rows per lane it is reduced once here
+    using EFac = typename std::conditional<R == 1, Pcr1Factor<G, JT>, Pcr2Factor<G, JT>>::type;
+    EFac efac;
+    if constexpr (R <= 2) {
+        JT ea[R], eb[R], ec[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            int r = gl * R + k;
+            bool last = r >= N - 1;
+            double wkn = (k + 1 < R) ? wk[k + 1] : g_down<G>(wk[0], 1);
+            double wku = last ? 0.0 : wkn;
+            ea[k] = (JT)(dt * wk[k]);
+            ec[k] = (JT)(dt * wku);
+            eb[k] = (JT)(-dt * (wk[k] + wku) - dzc[k] * rcs[k]);
+            if (!on[k]) {
+                ea[k] = 0;
+                eb[k] = 1;
+                ec[k] = 0;
+            }
+        }
+        if constexpr (R == 1)
+            efac.factor(gl, ea[0], eb[0], ec[0]);
+        else
+            efac.factor(gl, ea, eb, ec);
+    }
     const bool warm = gv != nullptr;
     int kit = 0, kit_total = 0;
     bool conv = false, fell_back = false;
@@ -502,7 +602,12 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             xv[k] -= dv[k];
             e2 = nan_max(e2, fabs(dv[k]));
         }
-        pcr_solve<G, R, JT>(gl, a, b, c, d, y);  // y = Δρe / ρc_s
+        if constexpr (R == 1)
+            y[0] = efac.solve(gl, d[0]);  // y = Δρe / ρc_s
+        else if constexpr (R == 2)
+            efac.solve(gl, d, y);
+        else
+            pcr_solve<G, R, JT>(gl, a, b, c, d, y);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             double yr = on[k] ? (double)y[k] : 0.0;

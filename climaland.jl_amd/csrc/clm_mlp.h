@@ -1,7 +1,7 @@
 // clm_mlp.h -- learned Newton warm start: a per-column MLP that predicts the
 // implicit part of the step's increment, x_{n+1} - U0, from the column state.
 //
-// Network: input features -> H -> H -> H -> 2x32 node outputs, GELU between
+// Network: input features -> H -> H -> H -> 2x32 node outputs, GELU (tanh form) between
 // layers, BF16 weights and activations, FP32 accumulation on MFMA
 // (v_mfma_f32_32x32x16_bf16).  Columns are resampled to 32 nodes in layer-
 // index space (node j sits at fractional layer (j + 0.5) N / 32 - 0.5, node 0
@@ -27,6 +27,7 @@ namespace ws {
 
 typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
 constexpr int kNodes = 32;
@@ -61,126 +62,168 @@ __device__ __forceinline__ double at_cell(int i, int N, const float* __restrict_
     return (1.0 - t) * (double)o[j0] + t * (double)o[j1];
 }
 
-// non-finite inputs (a column whose iterate blew up) enter the network as 0;
-// such columns are masked out of training and fall back to the plain start
-__device__ __forceinline__ bf16 feat(double x) { return (bf16)(float)(isfinite(x) ? x : 0.0); }
+// Features are O(1) by construction.  A column whose iterate blew up enters
+// the network as 0 (non-finite) or clamped to +-20 (finite but absurd), so
+// that no activation overflows: such columns are masked out of training
+// (a zero loss gradient times an overflowed activation would still be NaN)
+// and their guess is discarded when Newton falls back to the plain start.
+__device__ __forceinline__ bf16 feat(double x) {
+    return (bf16)(float)(isfinite(x) ? fmin(fmax(x, -20.0), 20.0) : 0.0);
+}
 
-// ---- features of the current state: one thread per column --------------------
+// The per-column kernels below give each column a group of 32 lanes (lane j
+// owns node j, or cells j, j+32, ...) so that rows of X, T and Y and the
+// column's cells are read and written contiguously.
+constexpr int kGroup = 32;
+
+__device__ __forceinline__ double tail_feature(int i, bool eh, double al, double n, double ks,
+                                               double nu, double thr, double fw, double fh,
+                                               double dt, int N) {
+    if (!eh && i >= 6) ++i;  // RichardsModel has no heat-flux feature
+    switch (i) {
+        case 0: return log10(al);
+        case 1: return n - 2.0;
+        case 2: return (log10(ks) + 5.5) / 1.5;
+        case 3: return (nu - 0.45) / 0.1;
+        case 4: return (thr - 0.06) / 0.04;
+        case 5: return fw / 1e-5;
+        case 6: return fh / 100.0;
+        case 7: return dt / 3600.0;
+        default: return (double)N / 40.0;
+    }
+}
+
+// ---- features of the current state: 32 lanes per column -----------------------
 __global__ __launch_bounds__(256) void k_features(
     bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
     const double* __restrict__ xv, const double* __restrict__ xi, const double* __restrict__ xe,
     const double* __restrict__ ftw, const double* __restrict__ fth, double dt, int Fp,
     bf16* __restrict__ X) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = gt / kGroup;
+    const int j = (int)(gt % kGroup);
     if (c >= ncol) return;
     const int64_t b = col_ptr[c];
     const int N = (int)(col_ptr[c + 1] - b);
     bf16* row = X + c * (int64_t)Fp;
     const double nu = par[P_NU * ncol + c], thr = par[P_THETA_R * ncol + c];
-    const double al = par[P_VG_ALPHA * ncol + c], n = par[P_VG_N * ncol + c];
-    const double ks = par[P_KSAT * ncol + c];
-    int f = 0;
-    for (int j = 0; j < kNodes; ++j)
-        row[f++] = feat((at_node(j, N, [&](int i) { return xv[b + i]; }) - thr) /
-                                 (nu - thr));
+    row[j] = feat((at_node(j, N, [&](int i) { return xv[b + i]; }) - thr) / (nu - thr));
+    int base = kNodes;
     if (eh) {
         const double rcd = par[P_RHOC_DS * ncol + c];
-        for (int j = 0; j < kNodes; ++j) {
-            double T = at_node(j, N, [&](int i) {
-                double v = xv[b + i], t = xi[b + i];
-                return temperature(xe[b + i], t, heat_capacity(fmin(nu - t, v), t, rcd));
-            });
-            row[f++] = feat((T - 273.15) / 10.0);
-        }
-        for (int j = 0; j < kNodes; ++j)
-            row[f++] = feat(at_node(j, N, [&](int i) { return xi[b + i]; }) / nu);
+        double T = at_node(j, N, [&](int i) {
+            double v = xv[b + i], t = xi[b + i];
+            return temperature(xe[b + i], t, heat_capacity(fmin(nu - t, v), t, rcd));
+        });
+        row[kNodes + j] = feat((T - 273.15) / 10.0);
+        row[2 * kNodes + j] = feat(at_node(j, N, [&](int i) { return xi[b + i]; }) / nu);
+        base = 3 * kNodes;
     }
-    row[f++] = feat(log10(al));
-    row[f++] = feat(n - 2.0);
-    row[f++] = feat((log10(ks) + 5.5) / 1.5);
-    row[f++] = feat((nu - 0.45) / 0.1);
-    row[f++] = feat((thr - 0.06) / 0.04);
-    row[f++] = feat(ftw[c] / 1e-5);
-    if (eh) row[f++] = feat(fth[c] / 100.0);
-    row[f++] = feat(dt / 3600.0);
-    row[f++] = feat((double)N / 40.0);
-    for (; f < Fp; ++f) row[f] = (bf16)0.0f;
+    const int ntail = eh ? 9 : 8;
+    for (int f = base + j; f < Fp; f += kGroup) {
+        int i = f - base;
+        row[f] = i < ntail ? feat(tail_feature(i, eh, par[P_VG_ALPHA * ncol + c],
+                                               par[P_VG_N * ncol + c], par[P_KSAT * ncol + c], nu,
+                                               thr, ftw[c], eh ? fth[c] : 0.0, dt, N))
+                           : (bf16)0.0f;
+    }
 }
 
-// ---- training targets from (u_n, U0_ϑ, x_{n+1}): one thread per column -------
-// mask = 1 for a column-step that converged with finite iterates
+// ---- training targets from (u_n, U0_ϑ, x_{n+1}): 32 lanes per column ----------
+// mask = 1 for a column-step that converged with finite iterates and a
+// target within 1e3 output units
 __global__ __launch_bounds__(256) void k_targets(
     bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
     const double* __restrict__ u0v, const double* __restrict__ unv, const double* __restrict__ uni,
     const double* __restrict__ une, const double* __restrict__ xv, const double* __restrict__ xe,
     const int32_t* __restrict__ flags, int Nout, float* __restrict__ T, float* __restrict__ mask) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
-    const int64_t b = col_ptr[c];
-    const int N = (int)(col_ptr[c + 1] - b);
-    float* row = T + c * (int64_t)Nout;
-    bool ok = (flags[c] & 0x70000) == 0;
-    for (int j = 0; j < kNodes; ++j) {
-        double d = at_node(j, N, [&](int i) { return xv[b + i] - u0v[b + i]; }) / kScaleV;
-        ok = ok && isfinite(d);
-        row[j] = (float)d;
-    }
-    if (eh) {
-        const double nu = par[P_NU * ncol + c], rcd = par[P_RHOC_DS * ncol + c];
-        for (int j = 0; j < kNodes; ++j) {
-            double d = at_node(j, N, [&](int i) {
+    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = gt / kGroup;
+    const int j = (int)(gt % kGroup);
+    const bool live = c < ncol;
+    bool ok = false;
+    double dv = 0.0, de = 0.0;
+    if (live) {
+        const int64_t b = col_ptr[c];
+        const int N = (int)(col_ptr[c + 1] - b);
+        ok = (flags[c] & 0x70000) == 0;
+        dv = at_node(j, N, [&](int i) { return xv[b + i] - u0v[b + i]; }) / kScaleV;
+        ok = ok && isfinite(dv) && fabs(dv) <= 1e3;
+        if (eh) {
+            const double nu = par[P_NU * ncol + c], rcd = par[P_RHOC_DS * ncol + c];
+            de = at_node(j, N, [&](int i) {
                 double v = unv[b + i], t = uni[b + i];
                 return (xe[b + i] - une[b + i]) / heat_capacity(fmin(nu - t, v), t, rcd);
             }) / kScaleT;
-            ok = ok && isfinite(d);
-            row[kNodes + j] = (float)d;
+            ok = ok && isfinite(de) && fabs(de) <= 1e3;
         }
     }
-    if (!ok)
-        for (int j = 0; j < Nout; ++j) row[j] = 0.0f;
-    mask[c] = ok ? 1.0f : 0.0f;
+    // the column is usable only if every node of its group is
+    unsigned long long bal = __ballot(ok);
+    unsigned long long gmask = 0xffffffffull << (threadIdx.x & 32);
+    bool all_ok = (bal & gmask) == gmask;
+    if (!live) return;
+    float* row = T + c * (int64_t)Nout;
+    row[j] = all_ok ? (float)dv : 0.0f;
+    if (eh) row[kNodes + j] = all_ok ? (float)de : 0.0f;
+    if (j == 0) mask[c] = all_ok ? 1.0f : 0.0f;
 }
 
-// ---- network output -> per-cell guess (FP64, CSR): one thread per column ------
+// ---- network output -> per-cell guess (FP64, CSR): 32 lanes per column ---------
 __global__ __launch_bounds__(256) void k_guess(
     bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
     const double* __restrict__ xv, const double* __restrict__ xi, const float* __restrict__ Y,
     int Nout, double* __restrict__ gv, double* __restrict__ ge) {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = gt / kGroup;
+    const int j = (int)(gt % kGroup);
     if (c >= ncol) return;
     const int64_t b = col_ptr[c];
     const int N = (int)(col_ptr[c + 1] - b);
     const float* o = Y + c * (int64_t)Nout;
-    for (int i = 0; i < N; ++i) gv[b + i] = at_cell(i, N, o) * kScaleV;
-    if (eh) {
-        const double nu = par[P_NU * ncol + c], rcd = par[P_RHOC_DS * ncol + c];
-        for (int i = 0; i < N; ++i) {
+    const double nu = eh ? par[P_NU * ncol + c] : 0.0;
+    const double rcd = eh ? par[P_RHOC_DS * ncol + c] : 0.0;
+    for (int i = j; i < N; i += kGroup) {
+        gv[b + i] = at_cell(i, N, o) * kScaleV;
+        if (eh) {
             double v = xv[b + i], t = xi[b + i];
-            ge[b + i] = at_cell(i, N, o + kNodes) * kScaleT *
-                        heat_capacity(fmin(nu - t, v), t, rcd);
+            ge[b + i] = at_cell(i, N, o + kNodes) * kScaleT * heat_capacity(fmin(nu - t, v), t, rcd);
         }
     }
 }
 
 // ---- GEMM: C[M x N] = A[M x K] . B[N x K]^T (both K-contiguous), BF16 in, FP32 acc
-// 128 x 128 tile per 256-thread block (4 waves of 64 x 64, 2 x 2 MFMA 32x32x16
-// tiles each), K in chunks of 32 staged through LDS.  K % 32 == 0 and lda,
-// ldb % 8 == 0 (16-byte loads).  blockIdx.z splits K into slices of kslice
-// (EPI_F32 only: slice z writes its partial product to Cf + z * M * ldcf).
+// 256 x 128 tile per 256-thread block: wave w owns rows 128 (w / 2) .. +128 and
+// columns 64 (w % 2) .. +64 as 4 x 2 MFMA 32x32x16 tiles (128 accumulators per
+// lane).  The MFMA computes each tile transposed (B fragment first), so a
+// lane's accumulators r = 4q..4q+3 are four consecutive columns of one row
+// and the epilogue stores them as one vector.  K advances in chunks of 32 staged through LDS; the next chunk's
+// global loads are issued before the current chunk's MFMAs (register
+// double buffering).  Requires K % 32 == 0 and lda, ldb % 8 == 0 (16-byte
+// loads); rows beyond M / N read as zero and are not stored.  blockIdx.z
+// splits K into slices of kslice (EPI_F32 only: slice z writes its partial
+// product to Cf + z * M * ldcf).
 enum Epi { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_DGELU = 3 };
 
+// GELU, tanh form: z Φ(z) ≈ z σ(2 sqrt(2/π) (z + 0.044715 z^3)), and its
+// derivative; σ from the hardware exp and reciprocal (a few ulp in FP32, far
+// below the BF16 rounding of the result)
+__device__ __forceinline__ float sigm(float u) {
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-u));
+}
 __device__ __forceinline__ float gelu(float z) {
-    return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+    return z * sigm(1.5957691216057308f * (z + 0.044715f * z * z * z));
 }
 __device__ __forceinline__ float dgelu(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-           z * 0.39894228040143268f * expf(-0.5f * z * z);
+    float z2 = z * z;
+    float sg = sigm(1.5957691216057308f * (z + 0.044715f * z2 * z));
+    return sg + z * sg * (1.0f - sg) * 1.5957691216057308f * (1.0f + 0.134145f * z2);
 }
 
-constexpr int kBM = 128, kBN = 128, kBK = 32, kLdsRow = kBK + 8;
+constexpr int kBM = 256, kBN = 128, kBK = 32, kLdsRow = kBK + 8;
 
 template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_nt(
+__global__ __launch_bounds__(256, 2) void k_gemm_nt(
     int M, int N, int K, int kslice, const bf16* __restrict__ A, int lda,
     const bf16* __restrict__ B, int ldb, const float* __restrict__ bias, float* __restrict__ Cf,
     int ldcf, bf16* __restrict__ Cb, int ldcb, const float* __restrict__ Z, int ldz) {
@@ -191,69 +234,117 @@ __global__ __launch_bounds__(256) void k_gemm_nt(
     const int m0 = blockIdx.x * kBM, n0 = blockIdx.y * kBN;
     const int kbeg = blockIdx.z * kslice;
     const int kend = min(K, kbeg + kslice);
-    floatx16 acc[2][2];
+    floatx16 acc[4][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
     const uint4 zero4 = {0u, 0u, 0u, 0u};
-    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+    uint4 ra[4], rb[2];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            int cidx = t + 256 * h;
+            int row = cidx >> 2, kq = (cidx & 3) * 8;
+            ra[h] = (m0 + row < M) ? *(const uint4*)(A + (int64_t)(m0 + row) * lda + k0 + kq)
+                                   : zero4;
+        }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             int cidx = t + 256 * h;
             int row = cidx >> 2, kq = (cidx & 3) * 8;
-            uint4 va = zero4, vb = zero4;
-            if (m0 + row < M) va = *(const uint4*)(A + (int64_t)(m0 + row) * lda + k0 + kq);
-            if (n0 + row < N) vb = *(const uint4*)(B + (int64_t)(n0 + row) * ldb + k0 + kq);
-            *(uint4*)(As + row * kLdsRow + kq) = va;
-            *(uint4*)(Bs + row * kLdsRow + kq) = vb;
+            rb[h] = (n0 + row < N) ? *(const uint4*)(B + (int64_t)(n0 + row) * ldb + k0 + kq)
+                                   : zero4;
         }
-        __syncthreads();
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            int cidx = t + 256 * h;
+            *(uint4*)(As + (cidx >> 2) * kLdsRow + (cidx & 3) * 8) = ra[h];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            int cidx = t + 256 * h;
+            *(uint4*)(Bs + (cidx >> 2) * kLdsRow + (cidx & 3) * 8) = rb[h];
+        }
+    };
+    if (kbeg < kend) {
+        gload(kbeg);
+        lstore();
+    }
+    __syncthreads();
+    for (int k0 = kbeg; k0 < kend; k0 += kBK) {
+        const bool more = k0 + kBK < kend;
+        if (more) gload(k0 + kBK);  // in flight during this chunk's MFMAs
 #pragma unroll
         for (int ks = 0; ks < kBK; ks += 16) {
-            bf16x8 a[2], b[2];
+            bf16x8 a[4], b[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-                a[i] = *(const bf16x8*)(As + (wm * 64 + i * 32 + (lane & 31)) * kLdsRow + ks +
+            for (int i = 0; i < 4; ++i)
+                a[i] = *(const bf16x8*)(As + (wm * 128 + i * 32 + (lane & 31)) * kLdsRow + ks +
                                         (lane >> 5) * 8);
 #pragma unroll
             for (int j = 0; j < 2; ++j)
                 b[j] = *(const bf16x8*)(Bs + (wn * 64 + j * 32 + (lane & 31)) * kLdsRow + ks +
                                         (lane >> 5) * 8);
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0,
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0,
                                                                         0, 0);
         }
         __syncthreads();
+        if (more) {
+            lstore();
+            __syncthreads();
+        }
     }
-    // D[row][col] of a 32x32 tile: col = lane % 32, row = 8 (r / 4) + 4 (lane / 32) + r % 4
+    // tile (i, j), accumulator r = 4q + e: row m = 32 i + lane % 32, column
+    // n = 32 j + 8 q + 4 (lane / 32) + e  (N % 4 == 0: a group of four is
+    // wholly inside or outside the matrix)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i) {
+        const int m = m0 + wm * 128 + i * 32 + (lane & 31);
+        if (m >= M) continue;
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                int col = n0 + wn * 64 + j * 32 + (lane & 31);
-                if (row >= M || col >= N) continue;
-                float v = acc[i][j][r];
+            for (int q = 0; q < 4; ++q) {
+                const int n = n0 + wn * 64 + j * 32 + 8 * q + 4 * (lane >> 5);
+                if (n >= N) continue;
+                float4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
+                            acc[i][j][4 * q + 3]};
                 if constexpr (EPI == EPI_F32) {
-                    Cf[(int64_t)blockIdx.z * M * ldcf + (int64_t)row * ldcf + col] = v;
+                    *(float4*)(Cf + (int64_t)blockIdx.z * M * ldcf + (int64_t)m * ldcf + n) = v;
                 } else if constexpr (EPI == EPI_BIAS_F32) {
-                    Cf[(int64_t)row * ldcf + col] = v + bias[col];
+                    float4 bb = *(const float4*)(bias + n);
+                    v.x += bb.x;
+                    v.y += bb.y;
+                    v.z += bb.z;
+                    v.w += bb.w;
+                    *(float4*)(Cf + (int64_t)m * ldcf + n) = v;
                 } else if constexpr (EPI == EPI_BIAS_GELU) {
-                    float z = v + bias[col];
-                    if (Cf) Cf[(int64_t)row * ldcf + col] = z;
-                    Cb[(int64_t)row * ldcb + col] = (bf16)gelu(z);
+                    float4 bb = *(const float4*)(bias + n);
+                    v.x += bb.x;
+                    v.y += bb.y;
+                    v.z += bb.z;
+                    v.w += bb.w;
+                    if (Cf) *(float4*)(Cf + (int64_t)m * ldcf + n) = v;
+                    bf16x4 h = {(bf16)gelu(v.x), (bf16)gelu(v.y), (bf16)gelu(v.z),
+                                (bf16)gelu(v.w)};
+                    *(bf16x4*)(Cb + (int64_t)m * ldcb + n) = h;
                 } else {
-                    Cb[(int64_t)row * ldcb + col] = (bf16)(v * dgelu(Z[(int64_t)row * ldz + col]));
+                    float4 z = *(const float4*)(Z + (int64_t)m * ldz + n);
+                    bf16x4 h = {(bf16)(v.x * dgelu(z.x)), (bf16)(v.y * dgelu(z.y)),
+                                (bf16)(v.z * dgelu(z.z)), (bf16)(v.w * dgelu(z.w))};
+                    *(bf16x4*)(Cb + (int64_t)m * ldcb + n) = h;
                 }
             }
+    }
 }
 
 // ---- small training kernels ---------------------------------------------------
@@ -284,14 +375,22 @@ __global__ __launch_bounds__(256) void k_sum_slices(int64_t n, int S, const floa
     out[i] = s;
 }
 
-// column sums of a bf16 [M x N] matrix (bias gradients)
+// partial column sums of a bf16 [M x N] matrix (bias gradients): block
+// (x, y) sums rows 256 y .. 256 y + 255 of columns 64 x .. 64 x + 63 into
+// part[y * N + n]; k_sum_slices then adds the gridDim.y partials
 __global__ __launch_bounds__(256) void k_colsum(int M, int N, const bf16* __restrict__ D,
-                                                float* __restrict__ out) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+                                                float* __restrict__ part) {
+    __shared__ float red[4][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + tx;
+    const int r0 = blockIdx.y * 256;
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += (float)D[(int64_t)m * N + n];
-    out[n] = s;
+    if (n < N)
+        for (int m = r0 + ty; m < min(M, r0 + 256); m += 4) s += (float)D[(int64_t)m * N + n];
+    red[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && n < N)
+        part[(int64_t)blockIdx.y * N + n] = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
 }
 
 // MSE loss sum(mask (Y - T)^2) / (M Nout) and its gradient dY (bf16)

@@ -588,8 +588,11 @@ void ws_wgrad(clm_ctx* c, WsState* w, int l, const bf16* D, const bf16* Hin, int
     int64_t nk = (int64_t)N * K;
     hipLaunchKernelGGL(ws::k_sum_slices, grid1(nk), dim3(256), 0, c->stream, nk, S, w->part,
                        w->g + w->off[l]);
-    hipLaunchKernelGGL(ws::k_colsum, grid1(N), dim3(256), 0, c->stream, M, N, D,
-                       w->g + w->off[l] + nk);
+    const int ry = (M + 255) / 256;
+    hipLaunchKernelGGL(ws::k_colsum, dim3((unsigned)((N + 63) / 64), (unsigned)ry), dim3(256), 0,
+                       c->stream, M, N, D, w->part);
+    hipLaunchKernelGGL(ws::k_sum_slices, grid1(N), dim3(256), 0, c->stream, (int64_t)N, ry,
+                       w->part, w->g + w->off[l] + nk);
 }
 
 // backward pass from dY (bf16 [M x Nout]) to every parameter gradient
@@ -651,7 +654,7 @@ int ws_batch_buffers(clm_ctx* c, WsState* w, int64_t M) {
 
 int ws_features_launch(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
                        bf16* X) {
-    hipLaunchKernelGGL(ws::k_features, grid1(c->ncol), dim3(256), 0, c->stream, is_eh(c),
+    hipLaunchKernelGGL(ws::k_features, grid1(c->ncol * ws::kGroup), dim3(256), 0, c->stream, is_eh(c),
                        c->d_col_ptr, c->ncol, c->d_cpar, c->d_cv, c->d_ci, c->d_ce, ftw,
                        fth ? fth : ftw, dt, w->d.Fp, X);
     return CLM_OK;
@@ -663,7 +666,7 @@ void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt
     if (!w || !w->enabled) return;
     ws_features_launch(c, w, ftw, fth, dt, w->X);
     ws_forward(c, w, w->X, (int)c->ncol, false, w->Y);
-    hipLaunchKernelGGL(ws::k_guess, grid1(c->ncol), dim3(256), 0, c->stream, is_eh(c),
+    hipLaunchKernelGGL(ws::k_guess, grid1(c->ncol * ws::kGroup), dim3(256), 0, c->stream, is_eh(c),
                        c->d_col_ptr, c->ncol, c->d_cpar, c->d_cv, c->d_ci, w->Y, w->d.Nout,
                        w->gv, w->ge);
     x->gv = w->gv;
@@ -1705,13 +1708,14 @@ int clm_ws_features(clm_ctx* c, double dt, float* out) {
 }
 
 int clm_ws_collect(clm_ctx* c, double dt, int32_t nsteps, int32_t steps_per_slot,
-                   int64_t* rows_out) {
-    if (!c || nsteps < 1 || steps_per_slot < 1 || !(dt > 0.0)) return CLM_E_ARG;
+                   int32_t stride, int64_t* rows_out) {
+    if (!c || nsteps < 1 || steps_per_slot < 1 || stride < 1 || !(dt > 0.0)) return CLM_E_ARG;
     if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
     if (c->tab_slots < 1) return set_err(c, CLM_E_STATE, "clm_set_forcing_table must run first");
     WsState* w = c->ws;
     const int Fp = w->d.Fp, No = w->d.Nout;
-    int64_t need = w->rows + (int64_t)nsteps * c->ncol;
+    const int32_t ncollect = (nsteps + stride - 1) / stride;
+    int64_t need = w->rows + (int64_t)ncollect * c->ncol;
     int r = 0;
     if (need > w->cap) {  // grow the training set, keeping its rows
         bf16* X2 = nullptr;
@@ -1747,6 +1751,10 @@ int clm_ws_collect(clm_ctx* c, double dt, int32_t nsteps, int32_t steps_per_slot
         int64_t slot = (s / steps_per_slot) % c->tab_slots;
         const double* ftw = c->d_tab_w + slot * c->ncol;
         const double* fth = eh ? c->d_tab_h + slot * c->ncol : nullptr;
+        if (s % stride != 0) {  // a plain step between two collected ones
+            launch_step(c, ftw, fth, dt, c->d_iters, nullptr, false);
+            continue;
+        }
         ws_features_launch(c, w, ftw, fth, dt, w->Xd + w->rows * Fp);
         HIPCHK(c, hipMemcpyAsync(w->unv, c->d_cv, 8 * (size_t)c->ncell, hipMemcpyDeviceToDevice,
                                  c->stream));
@@ -1757,7 +1765,7 @@ int clm_ws_collect(clm_ctx* c, double dt, int32_t nsteps, int32_t steps_per_slot
                                      hipMemcpyDeviceToDevice, c->stream));
         }
         launch_step(c, ftw, fth, dt, c->d_iters, w->u0v, false);
-        hipLaunchKernelGGL(ws::k_targets, grid1(c->ncol), dim3(256), 0, c->stream, eh,
+        hipLaunchKernelGGL(ws::k_targets, grid1(c->ncol * ws::kGroup), dim3(256), 0, c->stream, eh,
                            c->d_col_ptr, c->ncol, c->d_cpar, w->u0v, w->unv, w->uni, w->une,
                            c->d_cv, c->d_ce, c->d_iters, No, w->Td + w->rows * No,
                            w->Md + w->rows);

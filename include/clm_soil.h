@@ -242,10 +242,11 @@ int clm_ws_set_params(clm_ctx* ctx, const float* params);
 /* Use (1) or stop using (0) the warm start in clm_step / clm_run. */
 int clm_ws_enable(clm_ctx* ctx, int32_t on);
 /* Training pairs: advance the device state nsteps plain steps with the
- * clm_set_forcing_table slots and append (features of u_n, node targets of
- * x_{n+1} - U0, converged mask) per column-step; returns the row count. */
+ * clm_set_forcing_table slots and, for every stride-th step, append
+ * (features of u_n, node targets of x_{n+1} - U0, converged mask) per
+ * column; returns the total row count. */
 int clm_ws_collect(clm_ctx* ctx, double dt, int32_t nsteps, int32_t steps_per_slot,
-                   int64_t* rows);
+                   int32_t stride, int64_t* rows);
 /* Adam on the collected rows (masked MSE), minibatch `batch` (multiple of
  * 512), one loss value per epoch. */
 int clm_ws_train(clm_ctx* ctx, int32_t epochs, int32_t batch, float lr, uint64_t seed,

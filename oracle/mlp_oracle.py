@@ -5,8 +5,8 @@ TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the library's
 clm_ws_* kernels; never by the library's product path.
 
 The network mirrors the library's: BF16 inputs, weights and activations,
-sums accumulated in (here) float64, GELU(z) = z Φ(z), masked mean-square
-loss.  `bf16=False` drops every BF16 rounding, which gives the exact float64
+sums accumulated in (here) float64, GELU in its tanh form
+z σ(2 sqrt(2/π) (z + 0.044715 z^3)), masked mean-square loss.  `bf16=False` drops every BF16 rounding, which gives the exact float64
 function whose gradient finite differences pin.
 """
 from __future__ import annotations
@@ -14,7 +14,6 @@ from __future__ import annotations
 import math
 
 import numpy as np
-from scipy.special import erf
 
 NODES = 32
 SCALE_V = 1e-2
@@ -51,12 +50,18 @@ def unpack(params: np.ndarray, eh: bool):
     return out
 
 
+_C = 2.0 * math.sqrt(2.0 / math.pi)
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / math.sqrt(2.0)))
+    u = _C * (z + 0.044715 * z ** 3)
+    return z / (1.0 + np.exp(-u))
 
 
 def dgelu(z):
-    return 0.5 * (1.0 + erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2 * math.pi)
+    u = _C * (z + 0.044715 * z ** 3)
+    s = 1.0 / (1.0 + np.exp(-u))
+    return s + z * s * (1.0 - s) * _C * (1.0 + 3 * 0.044715 * z * z)
 
 
 def _r(x, bf16):
@@ -148,5 +153,6 @@ def features(batch, state, top_water, top_heat, dt, consts):
         tail += [dt / 3600.0, N / 40.0]
         r = np.concatenate(row + [np.array(tail)])
         assert r.size == F
-        X[c, :F] = np.where(np.isfinite(r), r, 0.0)  # non-finite inputs enter as 0
+        # non-finite inputs enter as 0, finite ones are clamped to [-20, 20]
+        X[c, :F] = np.clip(np.where(np.isfinite(r), r, 0.0), -20.0, 20.0)
     return X

@@ -57,8 +57,9 @@ def test_features_match_cpu(gpu_lib, name):
     m.close()
 
 
-@pytest.mark.parametrize("eh", [False, True])
-def test_network_forward_and_gradient_match_cpu(gpu_lib, eh):
+@pytest.mark.parametrize("eh,M", [(False, 256), (True, 256), (True, 2048)])
+def test_network_forward_and_gradient_match_cpu(gpu_lib, eh, M):
+    """M = 2048 takes the split-K path of the weight-gradient GEMMs."""
     from clmsoil import inputs
     from clmsoil.model import NewtonWarmStart
     from oracle import mlp_oracle as mo
@@ -68,7 +69,6 @@ def test_network_forward_and_gradient_match_cpu(gpu_lib, eh):
     F, Fp, H, No = mo.dims(eh)
     assert (ws.F, ws.Fp, ws.H, ws.Nout) == (F, Fp, H, No)
     rng = np.random.default_rng(11)
-    M = 256
     X = np.zeros((M, Fp), dtype=np.float32)
     X[:, :F] = rng.standard_normal((M, F))
     T = rng.standard_normal((M, No)).astype(np.float32)
