@@ -29,6 +29,12 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per step of the step kernels measured with rocprofv3 (FETCH_SIZE
+# and WRITE_SIZE summed over the bins' launches, FETCH scaled x2.0 and WRITE
+# x1.0 by a plain 1 GiB device copy); raw sums beside them.  Summary and
+# commands: profiles/k_eh_coop_eh_262144_uneven.md.
+MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 873.0e6, "raw": 531.3e6}}
+
 
 def step_bytes(batch, learned: bool) -> float:
     """Compulsory HBM bytes of one fused step over all columns (the group
@@ -296,7 +302,9 @@ def main():
                    "converged_fraction": 1.0 - (st.n_nonconverged / (batch.ncol * steps))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": args.traffic,
+                     "traffic": args.traffic if args.traffic is not None else
+                     MEASURED_TRAFFIC.get((w.name, args.form), {}).get("scaled"),
+                     "traffic_raw": MEASURED_TRAFFIC.get((w.name, args.form), {}).get("raw"),
                      "kernel": kname + " (one launch per depth bin; per step)",
                      "kernel_ms_per_step": r["kernel_ms"],
                      "bytes_per_step": bytes_step,
