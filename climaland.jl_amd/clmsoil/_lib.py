@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("CLMSOIL_LIB", os.path.join(PKG_ROOT, "libclmsoil.so")
 # mirrors of the enums in include/clm_soil.h
 CLM_OK, CLM_E_ARG, CLM_E_HIP, CLM_E_DOMAIN, CLM_E_STATE = 0, -1, -2, -3, -4
 MODEL_RICHARDS, MODEL_ENERGY_HYDROLOGY = 0, 1
-BC_FLUX, BC_FREE_DRAINAGE = 0, 1
+BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE = 0, 1, 2
 SOLVER_AUTO, SOLVER_GROUP, SOLVER_LANE = 0, 1, 2
 (P_NU, P_THETA_R, P_VG_ALPHA, P_VG_N, P_KSAT, P_SS, P_NU_SS_OM, P_NU_SS_QUARTZ,
  P_NU_SS_GRAVEL, P_KAPPA_DRY, P_KAPPA_SAT_FROZEN, P_KAPPA_SAT_UNFROZEN, P_RHOC_DS) = range(13)
@@ -43,7 +43,7 @@ class Config(C.Structure):
         ("model", C.c_int32), ("bc_bottom", C.c_int32), ("ncol", C.c_int64),
         ("col_ptr", _ip), ("z_face", _dp), ("params", _dp), ("nparam", C.c_int32),
         ("jac_precision", C.c_int32), ("newton_max_iters", C.c_int32),
-        ("newton_tol", C.c_double), ("solver", C.c_int32), ("reserved_", C.c_int32),
+        ("newton_tol", C.c_double), ("solver", C.c_int32), ("bc_top", C.c_int32),
     ]
 
 

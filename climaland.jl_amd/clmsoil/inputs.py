@@ -128,8 +128,10 @@ class ColumnBatch:
     rho_e_int: np.ndarray | None
     intF_water: np.ndarray
     intF_energy: np.ndarray | None
-    top_water: np.ndarray  # [nslots, ncol]
+    top_water: np.ndarray  # [nslots, ncol]; ϑ_bc under a top MoistureStateBC
     top_heat: np.ndarray | None
+    bc_top: int = L.BC_FLUX
+    bottom_water: np.ndarray | None = None  # [ncol]: bottom flux or ϑ_bc
 
     @property
     def ncol(self) -> int:
@@ -176,7 +178,26 @@ class ColumnBatch:
             self.rho_e_int[cell_idx].copy() if eh else None, self.intF_water[cols].copy(),
             self.intF_energy[cols].copy() if eh else None,
             np.ascontiguousarray(self.top_water[:, cols]),
-            np.ascontiguousarray(self.top_heat[:, cols]) if eh else None)
+            np.ascontiguousarray(self.top_heat[:, cols]) if eh else None, self.bc_top,
+            None if self.bottom_water is None else self.bottom_water[cols].copy())
+
+    def with_moisture_state(self, top_frac: float, bottom_frac: float | None = None,
+                            seed: int = 0) -> "ColumnBatch":
+        """The same RichardsModel columns under MoistureStateBC: ϑ_bc = θ_r +
+        s (ν - θ_r) with s ~ U[top_frac - 0.05, top_frac + 0.05] per column
+        at the top, and (if bottom_frac is given) likewise at the bottom;
+        the bottom stays free drainage otherwise."""
+        assert self.model == L.MODEL_RICHARDS
+        rng = np.random.default_rng(seed)
+        nu, thr = self.params[L.P_NU], self.params[L.P_THETA_R]
+        s_top = top_frac + rng.uniform(-0.05, 0.05, self.ncol)
+        top = np.broadcast_to(thr + s_top * (nu - thr), self.top_water.shape).copy()
+        b = dataclasses.replace(self, top_water=top, bc_top=L.BC_MOISTURE_STATE)
+        if bottom_frac is not None:
+            s_bot = bottom_frac + rng.uniform(-0.05, 0.05, self.ncol)
+            b = dataclasses.replace(b, bc_bottom=L.BC_MOISTURE_STATE,
+                                    bottom_water=thr + s_bot * (nu - thr))
+        return b
 
 
 def kappa_solid(nu_om, nu_q):

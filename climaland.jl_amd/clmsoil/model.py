@@ -112,6 +112,13 @@ class HeatFluxBC:
     bc: np.ndarray
 
 
+@dataclasses.dataclass
+class MoistureStateBC:
+    """Prescribed ϑ_l at the boundary, bc(p, t) per column
+    (boundary_conditions.jl:35-37, fluxes :227-325).  RichardsModel only."""
+    bc: np.ndarray
+
+
 class FreeDrainage:
     pass
 
@@ -147,7 +154,7 @@ class _SoilModel:
         cfg = L.Config(self.model_id, self._bottom_kind(), domain.ncol,
                        L.iptr(self._col_ptr), L.dptr(self._z_face), L.dptr(self._params),
                        self._params.shape[0], jac_precision, newton_max_iters, newton_tol,
-                       solver)
+                       solver, self._top_kind())
         L.check(self._lib, None, self._lib.clm_create(C.byref(self._ctx), C.byref(cfg), None))
         self.jac_precision = jac_precision
         self.set_forcing()
@@ -191,8 +198,14 @@ class _SoilModel:
 
     def _bottom_kind(self):
         b = self.boundary_conditions["bottom"]
+        if isinstance(b, MoistureStateBC):
+            return L.BC_MOISTURE_STATE
         return L.BC_FREE_DRAINAGE if isinstance(b, (FreeDrainage, EnergyWaterFreeDrainage)) \
             else L.BC_FLUX
+
+    def _top_kind(self):
+        top = self.boundary_conditions["top"]
+        return L.BC_MOISTURE_STATE if isinstance(top, MoistureStateBC) else L.BC_FLUX
 
     def set_forcing(self):
         """Push the current bc(p, t) values of the boundary conditions."""
@@ -205,7 +218,7 @@ class _SoilModel:
         if isinstance(b, WaterHeatBC):
             bw = np.ascontiguousarray(np.broadcast_to(b.water.bc, (n,)), float)
             bh = np.ascontiguousarray(np.broadcast_to(b.heat.bc, (n,)), float)
-        elif isinstance(b, WaterFluxBC):
+        elif isinstance(b, (WaterFluxBC, MoistureStateBC)):
             bw = np.ascontiguousarray(np.broadcast_to(b.bc, (n,)), float)
         self._check(self._lib.clm_set_forcing(self._ctx, L.dptr(tw), L.dptr(th), L.dptr(bw),
                                               L.dptr(bh)))
@@ -541,7 +554,12 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
               newton_tol=w.newton_tol if tol is None else tol, solver=solver)
     if batch.model == L.MODEL_RICHARDS:
         par = RichardsParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R])
-        bcs = {"top": WaterFluxBC(batch.top_water[slot]), "bottom": FreeDrainage()}
+        top = MoistureStateBC(batch.top_water[slot]) if batch.bc_top == L.BC_MOISTURE_STATE \
+            else WaterFluxBC(batch.top_water[slot])
+        bottom = {L.BC_FREE_DRAINAGE: FreeDrainage(),
+                  L.BC_MOISTURE_STATE: MoistureStateBC(batch.bottom_water),
+                  L.BC_FLUX: WaterFluxBC(batch.bottom_water)}[batch.bc_bottom]
+        bcs = {"top": top, "bottom": bottom}
         return RichardsModel(parameters=par, domain=dom, boundary_conditions=bcs, **kw)
     par = EnergyHydrologyParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R],
                                     P[L.P_NU_SS_OM], P[L.P_NU_SS_QUARTZ], P[L.P_NU_SS_GRAVEL],
@@ -556,6 +574,7 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
 __all__ = [
     "vanGenuchten", "RichardsParameters", "EnergyHydrologyParameters", "ColumnBatchDomain",
     "WaterFluxBC", "HeatFluxBC", "FreeDrainage", "EnergyWaterFreeDrainage", "WaterHeatBC",
+    "MoistureStateBC", "NewtonWarmStart",
     "RichardsModel", "EnergyHydrology", "make_update_cache", "make_update_implicit_cache",
     "make_exp_tendency", "make_compute_imp_tendency", "make_compute_jacobian",
     "initialize_jacobian", "LandSimulation", "model_from_batch",

@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
     double* __restrict__ xg, double* __restrict__ iw, const double* __restrict__ f_top,
-    const double* __restrict__ f_bot, int bc_bottom, double dt, int max_iters, double tol,
+    const double* __restrict__ f_bot, int bc, double dt, int max_iters, double tol,
     int32_t* __restrict__ iters, const double* __restrict__ gv, double* __restrict__ u0out) {
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
@@ -279,14 +279,24 @@ __global__ __launch_bounds__(256) void k_rre_coop(
 #pragma unroll
     for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);  // face below row
 
-    // lagged boundary fluxes from update_cache!(u_n)
+    // boundary fluxes from update_cache!(u_n); under a top MoistureStateBC
+    // they follow the iterate (rre.jl:460-468) and the top row carries the
+    // boundary term of set_dfluxBCdY! (boundary_conditions.jl:375-411)
+    const int bc_bottom = bc_bot(bc);
+    const bool top_state = bc_top(bc) == BC_MOISTURE_STATE;
     double bot;
     {
         PsiD p0 = psi_dpsi(v, nu, u0[0]);
         double K0 = on[0] ? hyd_cond_from_a(v, p0.S, p0.a) : 0.0;
-        double Kb = g_bcast<G>(K0, 0);
-        bot = (bc_bottom == BC_FREE_DRAINAGE) ? -1.0 * Kb : f_bot[col];
+        double Kb = g_bcast<G>(K0, 0), pb = g_bcast<G>(p0.psi, 0), db = g_bcast<G>(dzc[0], 0);
+        if (bc_bottom == BC_FREE_DRAINAGE)
+            bot = -1.0 * Kb;
+        else if (bc_bottom == BC_MOISTURE_STATE)
+            bot = moisture_flux_bot(v, nu, f_bot[col], Kb, pb, 0.5 * db);
+        else
+            bot = f_bot[col];
     }
+    double top_used = top;
     int kit = 0, kit_total = 0;
     bool conv = false, fell_back = false;
     double err = 0.0;
@@ -311,6 +321,12 @@ __global__ __launch_bounds__(256) void k_rre_coop(
             h[k] = p.psi + zc[k];
             D[k] = p.D;
         }
+        if (top_state && gl == 0 && bc_bottom != BC_FLUX) {  // bottom flux at the iterate
+            double psi0 = h[0] - zc[0];
+            bot = (bc_bottom == BC_FREE_DRAINAGE)
+                      ? -1.0 * K[0]
+                      : moisture_flux_bot(v, nu, f_bot[col], K[0], psi0, 0.5 * dzc[0]);
+        }
         // face below each row: conductance w and flux F
         double w[R], F[R];
 #pragma unroll
@@ -327,15 +343,21 @@ __global__ __launch_bounds__(256) void k_rre_coop(
             int r = gl * R + k;
             double wu = (k + 1 < R) ? w[k + 1] : g_down<G>(w[0], 1);
             double Fu = (k + 1 < R) ? F[k + 1] : g_down<G>(F[0], 1);
+            double wbc = 0.0;
             if (r >= N - 1) {
                 wu = 0.0;
                 Fu = top;
+                if (top_state && r == N - 1) {
+                    Fu = moisture_flux_top(v, nu, top, K[k], h[k] - zc[k], 0.5 * dzc[k]);
+                    wbc = K[k] / (0.5 * dzc[k]);
+                    top_used = Fu;
+                }
             }
             // Δz_c R = Δz_c (U0 - x) - Δt (F_up - F_down)
             double rhs = dzc[k] * (u0[k] - x[k]) - dt * (Fu - F[k]);
             a[k] = (JT)(dt * w[k]);
             c[k] = (JT)(dt * wu);
-            b[k] = (JT)(-dt * (w[k] + wu) - dzc[k] / D[k]);
+            b[k] = (JT)(-dt * (w[k] + wu + wbc) - dzc[k] / D[k]);
             d[k] = (JT)rhs;
             if (!on[k]) {
                 a[k] = 0;
@@ -362,8 +384,9 @@ __global__ __launch_bounds__(256) void k_rre_coop(
 #pragma unroll
     for (int k = 0; k < R; ++k)
         if (on[k]) xg[base + gl * R + k] = x[k];
+    if (top_state) top_used = g_bcast<G>(top_used, (N - 1) / R);
     if (gl == 0) {
-        iw[col] = iw[col] + dt * (-(top - bot));  // ∫F: -I block, exact
+        iw[col] = iw[col] + dt * (-(top_used - bot));  // ∫F: -I block, fluxes of the last iteration
         int32_t flag = kit_total;
         if (tol > 0.0 && !conv) flag |= 0x10000;
         if (!isfinite(err)) flag |= 0x20000;

@@ -14,7 +14,25 @@
 
 namespace clm {
 
-enum { BC_FLUX = 0, BC_FREE_DRAINAGE = 1 };
+enum { BC_FLUX = 0, BC_FREE_DRAINAGE = 1, BC_MOISTURE_STATE = 2 };
+// RichardsModel kernels take both boundary kinds packed as bottom | top << 4
+__host__ __device__ inline int bc_bot(int bc) { return bc & 15; }
+__host__ __device__ inline int bc_top(int bc) { return bc >> 4; }
+
+// MoistureStateBC (boundary_conditions.jl:227-325): diffusive_flux(K_eff,
+// x_2, x_1, Δz) = -K_eff (x_2 - x_1) / Δz with K_eff the boundary cell's K,
+// ψ_bc = pressure_head(ϑ_bc) with that cell's parameters, and gravity as
+// ψ + Δz on the upper point; Δz is the centre-to-boundary distance.
+__device__ __forceinline__ double moisture_flux_top(const VG& v, double nu, double theta_bc,
+                                                    double K_top, double psi_top, double dz) {
+    double psib = psi_dpsi(v, nu, theta_bc).psi;
+    return -K_top * ((psib + dz) - psi_top) / dz;
+}
+__device__ __forceinline__ double moisture_flux_bot(const VG& v, double nu, double theta_bc,
+                                                    double K_bot, double psi_bot, double dz) {
+    double psib = psi_dpsi(v, nu, theta_bc).psi;
+    return -K_bot * ((psi_bot + dz) - psib) / dz;
+}
 
 // per-slot parameter rows (same order as CLM_P_* in include/clm_soil.h)
 enum {
@@ -44,28 +62,48 @@ __device__ __forceinline__ VG load_vg(const double* __restrict__ par, int64_t ns
 
 // ---- fine-grained hooks -----------------------------------------------------
 
-// make_update_cache (rre.jl:368-380 update_aux + :111-149 boundary fluxes)
+// make_update_cache (rre.jl:368-380 update_aux + :111-149 boundary fluxes);
+// cache_imp! with do_bc = 0 (boundary fluxes lagged) or, under a top
+// MoistureStateBC, do_bc = 1 (rre.jl:460-468)
 __global__ __launch_bounds__(256) void k_rre_update_cache(
-    Layout L, const double* __restrict__ par, const double* __restrict__ vl,
-    double* __restrict__ K, double* __restrict__ psi, const double* __restrict__ f_top,
-    const double* __restrict__ f_bot, double* __restrict__ tw, double* __restrict__ bw,
-    int bc_bottom, int do_bc) {
+    Layout L, const double* __restrict__ par, const double* __restrict__ zf,
+    const double* __restrict__ vl, double* __restrict__ K, double* __restrict__ psi,
+    const double* __restrict__ f_top, const double* __restrict__ f_bot, double* __restrict__ tw,
+    double* __restrict__ bw, int bc, int do_bc) {
     ColView cv;
     if (!lane_column(L, cv)) return;
     VG v = load_vg(par, L.nslot, cv.s);
     double nu = par[P_NU * L.nslot + cv.s];
-    double K0 = 0.0;
+    double K0 = 0.0, p0 = 0.0, KN = 0.0, pN = 0.0;
     for (int j = 0; j < cv.N; ++j) {
         int64_t c = cv.cb + (int64_t)kWave * j;
         PsiD r = psi_dpsi(v, nu, vl[c]);
         double k = hyd_cond_from_a(v, r.S, r.a);
         K[c] = k;
         psi[c] = r.psi;
-        if (j == 0) K0 = k;
+        if (j == 0) {
+            K0 = k;
+            p0 = r.psi;
+        }
+        KN = k;
+        pN = r.psi;
     }
     if (do_bc) {
-        tw[cv.s] = f_top[cv.s];                                            // WaterFluxBC
-        bw[cv.s] = (bc_bottom == BC_FREE_DRAINAGE) ? -1.0 * K0 : f_bot[cv.s];  // FreeDrainage
+        const int N = cv.N;
+        if (bc_top(bc) == BC_MOISTURE_STATE) {
+            double dz = 0.5 * (zf[cv.fb + (int64_t)kWave * N] - zf[cv.fb + (int64_t)kWave * (N - 1)]);
+            tw[cv.s] = moisture_flux_top(v, nu, f_top[cv.s], KN, pN, dz);
+        } else {
+            tw[cv.s] = f_top[cv.s];  // WaterFluxBC
+        }
+        if (bc_bot(bc) == BC_FREE_DRAINAGE) {
+            bw[cv.s] = -1.0 * K0;
+        } else if (bc_bot(bc) == BC_MOISTURE_STATE) {
+            double dz = 0.5 * (zf[cv.fb + kWave] - zf[cv.fb]);
+            bw[cv.s] = moisture_flux_bot(v, nu, f_bot[cv.s], K0, p0, dz);
+        } else {
+            bw[cv.s] = f_bot[cv.s];
+        }
     }
 }
 
@@ -106,11 +144,14 @@ __global__ __launch_bounds__(256) void k_rre_imp_tendency(
 }
 
 // compute_jacobian! (rre.jl:391-458): rows (lo, di, up) of W
+// Under a top MoistureStateBC the top row's diagonal gains
+// -dtγ K_N dψ/dϑ_N / (Δz_c,N Δz_top) (set_dfluxBCdY!, boundary_conditions.jl:375-411,
+// added in rre.jl:434-447).
 template <typename JT>
 __global__ __launch_bounds__(256) void k_rre_jacobian(
     Layout L, const double* __restrict__ par, const double* __restrict__ zf,
     const double* __restrict__ vl, const double* __restrict__ K, double dtg,
-    JT* __restrict__ Wlo, JT* __restrict__ Wdi, JT* __restrict__ Wup) {
+    JT* __restrict__ Wlo, JT* __restrict__ Wdi, JT* __restrict__ Wup, int bc) {
     ColView cv;
     if (!lane_column(L, cv)) return;
     const int N = cv.N;
@@ -134,8 +175,9 @@ __global__ __launch_bounds__(256) void k_rre_jacobian(
         }
         double s = dtg / dzc_c;
         int64_t c = cv.cb + (int64_t)kWave * i;
+        double wbc = (i == N - 1 && bc_top(bc) == BC_MOISTURE_STATE) ? K_c / (0.5 * dzc_c) : 0.0;
         Wlo[c] = (JT)(s * wlo * D_m);
-        Wdi[c] = (JT)(-s * (wlo + wup) * D_c - 1.0);
+        Wdi[c] = (JT)(-s * (wlo + wup + wbc) * D_c - 1.0);
         Wup[c] = (JT)(s * wup * D_p);
         D_m = D_c;
         D_c = D_p;
@@ -202,11 +244,13 @@ __global__ __launch_bounds__(256) void k_rre_solve(Layout L, const JT* __restric
 // RRE explicit tendency is zero, rre.jl:214-243).
 //
 // iters[s] = Newton iterations | 0x10000 if not converged | 0x20000 if non-finite
+// Under a top MoistureStateBC both boundary fluxes follow the iterate
+// (rre.jl:460-468) and the top row carries the boundary term.
 template <typename JT>
 __global__ __launch_bounds__(256) void k_rre_step(
     Layout L, const double* __restrict__ par, const double* __restrict__ zf,
     double* __restrict__ x, double* __restrict__ U0, double* __restrict__ iw,
-    const double* __restrict__ f_top, const double* __restrict__ f_bot, int bc_bottom,
+    const double* __restrict__ f_top, const double* __restrict__ f_bot, int bc,
     double dt, int max_iters, double tol, JT* __restrict__ cp, JT* __restrict__ dp,
     int32_t* __restrict__ iters) {
     ColView cv;
@@ -215,8 +259,12 @@ __global__ __launch_bounds__(256) void k_rre_step(
     const int64_t cb = cv.cb, fb = cv.fb;
     VG v = load_vg(par, L.nslot, cv.s);
     const double nu = par[P_NU * L.nslot + cv.s];
-    const double top = f_top[cv.s];
-    double bot = (bc_bottom == BC_FREE_DRAINAGE) ? 0.0 : f_bot[cv.s];
+    const int bc_bottom = bc_bot(bc);
+    const bool top_state = bc_top(bc) == BC_MOISTURE_STATE;
+    const double ftop = f_top[cv.s];
+    double top = ftop;
+    double bot = (bc_bottom == BC_FLUX) ? f_bot[cv.s] : 0.0;
+    const double iw0 = iw[cv.s];
 
     int k = 0;
     bool conv = false;
@@ -229,7 +277,12 @@ __global__ __launch_bounds__(256) void k_rre_step(
         double x_c = x[cb];
         PsiD r = psi_dpsi(v, nu, x_c);
         double K_c = hyd_cond_from_a(v, r.S, r.a), D_c = r.D, h_c = r.psi + zc_c;
-        if (k == 1 && bc_bottom == BC_FREE_DRAINAGE) bot = -1.0 * K_c;  // -K(u_n) at level 1
+        if (k == 1 || top_state) {  // bottom flux at u_n, or at the iterate under a top state BC
+            if (bc_bottom == BC_FREE_DRAINAGE)
+                bot = -1.0 * K_c;
+            else if (bc_bottom == BC_MOISTURE_STATE)
+                bot = moisture_flux_bot(v, nu, f_bot[cv.s], K_c, r.psi, 0.5 * dzc_c);
+        }
         double Flo = bot, wlo = 0.0, D_m = 0.0;
         JT cpp = 0, dpp = 0;
         for (int i = 0; i < N; ++i) {
@@ -251,6 +304,7 @@ __global__ __launch_bounds__(256) void k_rre_step(
                 Fhi = -Kf * (h_n - h_c) * idzf;
                 f1 = f2;
             } else {
+                if (top_state) top = moisture_flux_top(v, nu, ftop, K_c, h_c - zc_c, 0.5 * dzc_c);
                 Fhi = top;
             }
             double idzc = 1.0 / dzc_c;
@@ -258,8 +312,9 @@ __global__ __launch_bounds__(256) void k_rre_step(
             if (k == 1) U0[c] = u0c;
             double R = u0c + dt * (-(Fhi - Flo) * idzc) - x_c;
             double s = dt * idzc;
+            double wbc = (i == N - 1 && top_state) ? K_c / (0.5 * dzc_c) : 0.0;
             JT lo = (JT)(s * wlo * D_m);
-            JT di = (JT)(-s * (wlo + wup) * D_c - 1.0);
+            JT di = (JT)(-s * (wlo + wup + wbc) * D_c - 1.0);
             JT up = (JT)(s * wup * D_n);
             JT inv = (JT)1 / (di - lo * cpp);
             cpp = up * inv;
@@ -287,7 +342,7 @@ __global__ __launch_bounds__(256) void k_rre_step(
             x[c] = x[c] - d;
             err = nan_max(err, fabs(d));
         }
-        if (k == 1) iw[cv.s] = iw[cv.s] + dt * (-(top - bot));  // ∫F: -I block, exact at k=1
+        iw[cv.s] = iw0 + dt * (-(top - bot));  // ∫F: -I block, fluxes of this iteration
         conv = (tol > 0.0) && (err < tol);
         // a non-finite iterate stays non-finite: stop (the state is NaN either way)
         if (conv || !isfinite(err)) break;

@@ -33,7 +33,7 @@ struct DevStats {
 }  // namespace
 
 struct clm_ctx {
-    int model = 0, bc_bottom = 0, jac_prec = 64, max_iters = 3;
+    int model = 0, bc_bottom = 0, bc_top = 0, jac_prec = 64, max_iters = 3;
     double tol = 0.0;
     int64_t ncol = 0, ncell = 0;
     int nparam = 0;
@@ -144,6 +144,8 @@ void dfree(T*& p) {
 }
 
 bool is_eh(const clm_ctx* c) { return c->model == CLM_MODEL_ENERGY_HYDROLOGY; }
+// RichardsModel kernels take both boundary kinds as bottom | top << 4
+int rre_bc(const clm_ctx* c) { return c->bc_bottom | (c->bc_top << 4); }
 bool is_group(const clm_ctx* c) { return c->solver == CLM_SOLVER_GROUP; }
 // length of per-column device vectors of the coarse path
 int64_t ncolvec(const clm_ctx* c) { return is_group(c) ? c->ncol : c->nslot; }
@@ -384,7 +386,7 @@ void launch_step_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
         JT* cp = (JT*)c->d_scr;
         JT* dp = cp + c->npcell;
         hipLaunchKernelGGL(k_rre_step<JT>, g, b, 0, c->stream, L, c->d_par, c->d_zf, c->d_xv,
-                           c->d_U0v, c->d_iw, ftw, c->d_fbw, c->bc_bottom, dt, c->max_iters,
+                           c->d_U0v, c->d_iw, ftw, c->d_fbw, rre_bc(c), dt, c->max_iters,
                            c->tol, cp, dp, iters);
     } else {
         hipLaunchKernelGGL(k_eh_step<JT>, g, b, 0, c->stream, L, c->d_par, c->d_zf,
@@ -410,7 +412,7 @@ void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const dou
     if (!is_eh(c)) {
         hipLaunchKernelGGL((k_rre_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
                            bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ciw, ftw, c->d_cfbw,
-                           c->bc_bottom, dt, c->max_iters, c->tol, iters, x.gv, x.u0out);
+                           rre_bc(c), dt, c->max_iters, c->tol, iters, x.gv, x.u0out);
     } else {
         hipLaunchKernelGGL((k_eh_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
                            bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci, c->d_ce,
@@ -716,8 +718,14 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     *out = nullptr;
     if (cfg->model != CLM_MODEL_RICHARDS && cfg->model != CLM_MODEL_ENERGY_HYDROLOGY)
         return set_err(nullptr, CLM_E_ARG, "unknown model %d", cfg->model);
-    if (cfg->bc_bottom != CLM_BC_FLUX && cfg->bc_bottom != CLM_BC_FREE_DRAINAGE)
+    if (cfg->bc_bottom != CLM_BC_FLUX && cfg->bc_bottom != CLM_BC_FREE_DRAINAGE &&
+        cfg->bc_bottom != CLM_BC_MOISTURE_STATE)
         return set_err(nullptr, CLM_E_ARG, "unknown bottom boundary %d", cfg->bc_bottom);
+    if (cfg->bc_top != CLM_BC_FLUX && cfg->bc_top != CLM_BC_MOISTURE_STATE)
+        return set_err(nullptr, CLM_E_ARG, "unknown top boundary %d", cfg->bc_top);
+    if (cfg->model == CLM_MODEL_ENERGY_HYDROLOGY &&
+        (cfg->bc_top == CLM_BC_MOISTURE_STATE || cfg->bc_bottom == CLM_BC_MOISTURE_STATE))
+        return set_err(nullptr, CLM_E_ARG, "MoistureStateBC is implemented for RichardsModel only");
     int need = cfg->model == CLM_MODEL_RICHARDS ? CLM_NPARAM_RRE : CLM_NPARAM_EH;
     if (cfg->nparam < need)
         return set_err(nullptr, CLM_E_ARG, "nparam %d < %d required", cfg->nparam, need);
@@ -757,6 +765,7 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     c->solver = solver;
     c->model = cfg->model;
     c->bc_bottom = cfg->bc_bottom;
+    c->bc_top = cfg->bc_top;
     c->jac_prec = cfg->jac_precision;
     c->max_iters = cfg->newton_max_iters;
     c->tol = cfg->newton_tol;
@@ -1117,8 +1126,8 @@ int clm_update_cache(clm_ctx* c, const clm_state* Y, double t) {
     Layout L = layout(c);
     if (!is_eh(c)) {
         hipLaunchKernelGGL(k_rre_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
-                           c->d_par, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw, c->d_tw,
-                           c->d_bw, c->bc_bottom, 1);
+                           c->d_par, c->d_zf, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw,
+                           c->d_tw, c->d_bw, rre_bc(c), 1);
     } else {
         hipLaunchKernelGGL(k_eh_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_par, c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_thl, c->d_kap,
@@ -1139,10 +1148,12 @@ int clm_update_implicit_cache(clm_ctx* c, const clm_state* Y, double t) {
     if (r) return r;
     Layout L = layout(c);
     if (!is_eh(c)) {
-        // update_aux (models.jl:207-210); boundary fluxes stay lagged (rre.jl:460-468)
+        // update_aux (models.jl:207-210); boundary fluxes stay lagged unless
+        // the top is a MoistureStateBC (rre.jl:460-468)
         hipLaunchKernelGGL(k_rre_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
-                           c->d_par, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw, c->d_tw,
-                           c->d_bw, c->bc_bottom, 0);
+                           c->d_par, c->d_zf, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw,
+                           c->d_tw, c->d_bw, rre_bc(c),
+                           c->bc_top == CLM_BC_MOISTURE_STATE ? 1 : 0);
     } else {
         hipLaunchKernelGGL(k_eh_imp_cache, grid_slots(c), dim3(256), 0, c->stream, L, c->d_par,
                            c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_T, c->d_psi);
@@ -1217,12 +1228,12 @@ int clm_jacobian(clm_ctx* c, const clm_state* Y, double dtgamma, double t) {
             float* W = (float*)c->d_W;
             hipLaunchKernelGGL(k_rre_jacobian<float>, grid_slots(c), dim3(256), 0, c->stream, L,
                                c->d_par, c->d_zf, c->d_wy[0], c->d_K, dtgamma, W, W + n,
-                               W + 2 * n);
+                               W + 2 * n, rre_bc(c));
         } else {
             double* W = (double*)c->d_W;
             hipLaunchKernelGGL(k_rre_jacobian<double>, grid_slots(c), dim3(256), 0, c->stream,
                                L, c->d_par, c->d_zf, c->d_wy[0], c->d_K, dtgamma, W, W + n,
-                               W + 2 * n);
+                               W + 2 * n, rre_bc(c));
         }
     } else {
         if (c->jac_prec == 32) {

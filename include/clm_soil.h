@@ -59,13 +59,19 @@ enum {
 
 enum { CLM_MODEL_RICHARDS = 0, CLM_MODEL_ENERGY_HYDROLOGY = 1 };
 
-/* Top boundary: prescribed fluxes (WaterFluxBC; EH: WaterHeatBC(WaterFluxBC,
- * HeatFluxBC)), boundary_conditions.jl:168-180, 465-477, 709-724.
- * Bottom boundary: */
+/* Water boundary conditions (boundary_conditions.jl).
+ * Top:    CLM_BC_FLUX (WaterFluxBC, :168-180; EH: WaterHeatBC(WaterFluxBC,
+ *         HeatFluxBC), :465-477, 709-724) or CLM_BC_MOISTURE_STATE (RRE).
+ * Bottom: CLM_BC_FLUX, CLM_BC_FREE_DRAINAGE or CLM_BC_MOISTURE_STATE (RRE).
+ * For CLM_BC_MOISTURE_STATE (MoistureStateBC, :227-325) the forcing value is
+ * the boundary ϑ_l instead of a flux; with a top MoistureStateBC both
+ * boundary fluxes follow the Newton iterate (rre.jl:460-468) and the
+ * Jacobian's top row carries set_dfluxBCdY!'s term (:375-411). */
 enum {
-    CLM_BC_FLUX = 0,          /* prescribed fluxes */
-    CLM_BC_FREE_DRAINAGE = 1  /* FreeDrainage (RRE, :340-353) or
-                                 EnergyWaterFreeDrainage (EH, :590-608) */
+    CLM_BC_FLUX = 0,           /* prescribed fluxes */
+    CLM_BC_FREE_DRAINAGE = 1,  /* FreeDrainage (RRE, :340-353) or
+                                  EnergyWaterFreeDrainage (EH, :590-608) */
+    CLM_BC_MOISTURE_STATE = 2  /* MoistureStateBC (RRE) */
 };
 
 /* Per-column parameter rows.  RRE uses the first CLM_NPARAM_RRE rows
@@ -103,7 +109,7 @@ typedef struct clm_ctx clm_ctx;
 
 typedef struct {
     int32_t model;            /* CLM_MODEL_* */
-    int32_t bc_bottom;        /* CLM_BC_* */
+    int32_t bc_bottom;        /* CLM_BC_* (bottom) */
     int64_t ncol;
     const int32_t* col_ptr;   /* ncol+1, col_ptr[0] = 0 */
     const double* z_face;     /* col_ptr[ncol] + ncol face heights, increasing */
@@ -116,7 +122,7 @@ typedef struct {
                                  > 0: per-column stop when max|Δx| < tol over
                                  the cell unknowns (ρe_int scaled by 1/ρc_s) */
     int32_t solver;           /* CLM_SOLVER_*: how clm_step / clm_run solve */
-    int32_t reserved_;
+    int32_t bc_top;           /* CLM_BC_FLUX or CLM_BC_MOISTURE_STATE */
 } clm_config;
 
 /* Prognostic state Y.soil.  RRE: vartheta_l, intF_water.

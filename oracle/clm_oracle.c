@@ -348,12 +348,45 @@ static void rre_aux(const col_geom* g, const col_par* q, const double* vl, doubl
     }
 }
 
-/* rre.jl:111-149 + boundary_conditions.jl:168-180 (WaterFluxBC),
- * :340-353 (FreeDrainage: -K at level 1) */
-static void rre_bc(int bc_bottom, const double* K, double f_top, double f_bot,
-                   double* tw, double* bw) {
-    *tw = f_top;
-    *bw = (bc_bottom == ORC_BC_FREE_DRAINAGE) ? -1.0 * K[0] : f_bot;
+/* distance from the top / bottom cell centre to the boundary face
+ * (Domains.jl:935-949 Δz_top, Δz_bottom) */
+static inline double dztop(const col_geom* g) { return g->zf[g->N] - zc(g, g->N - 1); }
+static inline double dzbot(const col_geom* g) { return zc(g, 0) - g->zf[0]; }
+
+/* rre.jl:111-149 boundary fluxes: WaterFluxBC (boundary_conditions.jl:168-180),
+ * FreeDrainage (:340-353, -K at level 1) and MoistureStateBC (:227-325):
+ * diffusive_flux(K_eff, x_2, x_1, Δz) = -K_eff (x_2 - x_1) / Δz with K_eff the
+ * boundary cell's K, ψ_bc = pressure_head(ϑ_bc) with that cell's parameters,
+ * and gravity entering as ψ + Δz on the upper point. */
+static void rre_bc(const orc_model* m, const col_geom* g, const col_par* q, const double* K,
+                   const double* psi, double f_top, double f_bot, double* tw, double* bw) {
+    int N = g->N;
+    if (m->bc_top == ORC_BC_MOISTURE_STATE) {
+        double dz = dztop(g);
+        double psib = orc_pressure_head(q->alpha, q->n, q->theta_r, f_top, q->nu, q->Ss);
+        *tw = -K[N - 1] * ((psib + dz) - psi[N - 1]) / dz;
+    } else {
+        *tw = f_top;
+    }
+    if (m->bc_bottom == ORC_BC_FREE_DRAINAGE) {
+        *bw = -1.0 * K[0];
+    } else if (m->bc_bottom == ORC_BC_MOISTURE_STATE) {
+        double dz = dzbot(g);
+        double psib = orc_pressure_head(q->alpha, q->n, q->theta_r, f_bot, q->nu, q->Ss);
+        *bw = -K[0] * ((psi[0] + dz) - psib) / dz;
+    } else {
+        *bw = f_bot;
+    }
+}
+
+/* set_dfluxBCdY! for a top MoistureStateBC (boundary_conditions.jl:375-411,
+ * added in rre.jl:434-447): the top row of W gains
+ * -dtγ K_N dψ/dϑ(ϑ_N) / (Δz_c,N Δz_top) on its diagonal. */
+static void rre_top_bc_jacobian(const orc_model* m, const col_geom* g, const double* K,
+                                const double* D, double dtg, double* W) {
+    if (m->bc_top != ORC_BC_MOISTURE_STATE) return;
+    int N = g->N;
+    W[3 * (N - 1) + 1] -= dtg * K[N - 1] * D[N - 1] / (dzc(g, N - 1) * dztop(g));
 }
 
 /* Darcy face flux of the implicit tendency, rre.jl:191-192:
@@ -605,14 +638,16 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
     double tw, bw;
     /* stage 1 */
     rre_aux(&g, &q, Y->vl, K, psi);
-    rre_bc(m->bc_bottom, K, f_top, f_bot, &tw, &bw);
+    rre_bc(m, &g, &q, K, psi, f_top, f_bot, &tw, &bw);
+    const int imp_bc = m->bc_top == ORC_BC_MOISTURE_STATE;
     /* exp tendency is zero (rre.jl:214-243): U0 = u_n */
     memcpy(U0, Y->vl, sizeof(double) * N);
     double U0iw = *Y->iw;
     double* x = Y->vl;
-    if (gv) { /* warm start x0 = U0 + guess; cache_imp!(x0), BC fluxes stay at u_n */
+    if (gv) { /* warm start x0 = U0 + guess; cache_imp!(x0) */
         for (int i = 0; i < N; ++i) x[i] += gv[i];
         rre_aux(&g, &q, x, K, psi);
+        if (imp_bc) rre_bc(m, &g, &q, K, psi, f_top, f_bot, &tw, &bw);
     }
     int k;
     for (k = 1; k <= max_iters; ++k) {
@@ -620,6 +655,7 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
         for (int i = 0; i < N; ++i)
             D[i] = orc_dpsidtheta(q.alpha, q.n, x[i], q.nu, q.theta_r, q.Ss);
         picard_tridiag(&g, K, D, dt, W);
+        rre_top_bc_jacobian(m, &g, K, D, dt, W);
         /* residual */
         double diw;
         rre_imp_tendency(&g, K, psi, tw, bw, R, &diw);
@@ -631,14 +667,17 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
             x[i] -= dx[i];
             err = nan_max(err, fabs(dx[i]));
         }
-        /* ∫F block is -I (implicit_timestepping.jl:149-152): Δ = -R.  It is
-         * exact after the first iteration (lagged fluxes), so the
+        /* ∫F block is -I (implicit_timestepping.jl:149-152): Δ = -R.  With
+         * lagged boundary fluxes it is exact after the first iteration; the
          * convergence norm runs over the cell unknowns only. */
         double dxiw = -Riw;
         *Y->iw -= dxiw;
         int conv = (tol > 0.0) && (err < tol);
-        /* cache_imp!(x): update_aux (models.jl:207-210); BCs stay lagged */
+        /* cache_imp!(x): update_aux (models.jl:207-210); the boundary
+         * fluxes are refreshed only under a top MoistureStateBC
+         * (rre.jl:460-468), otherwise they stay lagged at u_n */
         rre_aux(&g, &q, x, K, psi);
+        if (imp_bc) rre_bc(m, &g, &q, K, psi, f_top, f_bot, &tw, &bw);
         if (conv) break;
     }
     if (k > max_iters) {
@@ -772,7 +811,8 @@ int orc_update_cache(const orc_model* m, const orc_state* Y, const orc_forcing* 
         load_par(m, c, &q);
         if (m->model == ORC_MODEL_RICHARDS) {
             rre_aux(&g, &q, y.vl, a.K, a.psi);
-            rre_bc(m->bc_bottom, a.K, fval(f->top_water, c), fval(f->bot_water, c), a.tw, a.bw);
+            rre_bc(m, &g, &q, a.K, a.psi, fval(f->top_water, c), fval(f->bot_water, c), a.tw,
+                   a.bw);
         } else {
             eh_aux(&g, &q, y.vl, y.ti, y.re, a.thl, a.kap, a.T, a.K, a.psi);
             eh_bc(m->bc_bottom, a.K, a.T, fval(f->top_water, c), fval(f->top_heat, c),
@@ -782,7 +822,8 @@ int orc_update_cache(const orc_model* m, const orc_state* Y, const orc_forcing* 
     return 0;
 }
 
-int orc_update_implicit_cache(const orc_model* m, const orc_state* Y, orc_aux* p) {
+int orc_update_implicit_cache(const orc_model* m, const orc_state* Y, const orc_forcing* f,
+                              orc_aux* p) {
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t c = 0; c < m->ncol; ++c) {
         col_state y;
@@ -793,10 +834,14 @@ int orc_update_implicit_cache(const orc_model* m, const orc_state* Y, orc_aux* p
         col_geom g = {m->col_ptr[c + 1] - o, m->z_face + o + c};
         col_par q;
         load_par(m, c, &q);
-        if (m->model == ORC_MODEL_RICHARDS)
+        if (m->model == ORC_MODEL_RICHARDS) {
             rre_aux(&g, &q, y.vl, a.K, a.psi);
-        else
+            if (m->bc_top == ORC_BC_MOISTURE_STATE && f)
+                rre_bc(m, &g, &q, a.K, a.psi, fval(f->top_water, c), fval(f->bot_water, c),
+                       a.tw, a.bw);
+        } else {
             eh_imp_aux(&g, &q, y.vl, y.ti, y.re, a.T, a.psi);
+        }
     }
     return 0;
 }
@@ -865,6 +910,7 @@ int orc_jacobian(const orc_model* m, const orc_state* Y, const orc_aux* p, doubl
                 for (int i = 0; i < g.N; ++i)
                     t[i] = orc_dpsidtheta(q.alpha, q.n, y.vl[i], q.nu, q.theta_r, q.Ss);
                 picard_tridiag(&g, a.K, t, dtgamma, W_ww + 3 * (int64_t)o);
+                rre_top_bc_jacobian(m, &g, a.K, t, dtgamma, W_ww + 3 * (int64_t)o);
             } else {
                 eh_jacobian(&g, &q, y.vl, y.ti, a.K, a.T, a.kap, a.thl, dtgamma,
                             W_ww + 3 * (int64_t)o, W_ee + 3 * (int64_t)o,

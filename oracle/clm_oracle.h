@@ -40,10 +40,12 @@ enum {
 #define ORC_NPARAM_RRE 6
 
 enum { ORC_MODEL_RICHARDS = 0, ORC_MODEL_ENERGY_HYDROLOGY = 1 };
-/* bottom boundary: prescribed flux (WaterFluxBC / WaterHeatBC of fluxes) or
- * FreeDrainage (RRE) / EnergyWaterFreeDrainage (EH).  Top is always a
- * prescribed flux (WaterFluxBC, and HeatFluxBC for EH). */
-enum { ORC_BC_FLUX = 0, ORC_BC_FREE_DRAINAGE = 1 };
+/* Water boundary conditions.  Bottom: prescribed flux (WaterFluxBC /
+ * WaterHeatBC of fluxes), FreeDrainage (RRE) / EnergyWaterFreeDrainage (EH),
+ * or MoistureStateBC (RRE).  Top: prescribed flux (WaterFluxBC, with
+ * HeatFluxBC for EH) or MoistureStateBC (RRE).  For a MoistureStateBC the
+ * forcing value is the boundary ϑ_l, not a flux. */
+enum { ORC_BC_FLUX = 0, ORC_BC_FREE_DRAINAGE = 1, ORC_BC_MOISTURE_STATE = 2 };
 
 typedef struct {
     int32_t model;
@@ -52,6 +54,8 @@ typedef struct {
     const int32_t* col_ptr;
     const double* z_face;
     const double* params;
+    int32_t bc_top; /* ORC_BC_FLUX or ORC_BC_MOISTURE_STATE (RRE) */
+    int32_t reserved_;
 } orc_model;
 
 /* prognostic state Y (EH uses all five; RRE uses vartheta_l, intF_water) */
@@ -124,7 +128,11 @@ double orc_heaviside(double x);
 int orc_update_cache(const orc_model* m, const orc_state* Y, const orc_forcing* f,
                      orc_aux* p);
 /* make_update_implicit_cache: implicit aux (+ implicit BC refresh: none here) */
-int orc_update_implicit_cache(const orc_model* m, const orc_state* Y, orc_aux* p);
+/* cache_imp!: implicit aux; with a top MoistureStateBC (RRE) also the
+ * boundary fluxes (rre.jl:460-468), which need the forcing f (may be NULL
+ * otherwise) */
+int orc_update_implicit_cache(const orc_model* m, const orc_state* Y, const orc_forcing* f,
+                              orc_aux* p);
 /* compute_exp_tendency (p must be current) */
 int orc_exp_tendency(const orc_model* m, const orc_state* Y, const orc_aux* p,
                      orc_state* dY);

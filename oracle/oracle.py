@@ -27,7 +27,11 @@ CONSTANT_NAMES = ("RHO_L", "RHO_I", "CP_L", "CP_I", "T_0", "T_FREEZE", "LH_V0", 
 
 class Model(C.Structure):
     _fields_ = [("model", C.c_int32), ("bc_bottom", C.c_int32), ("ncol", C.c_int64),
-                ("col_ptr", _ip), ("z_face", _dp), ("params", _dp)]
+                ("col_ptr", _ip), ("z_face", _dp), ("params", _dp), ("bc_top", C.c_int32),
+                ("reserved_", C.c_int32)]
+
+
+BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE = 0, 1, 2
 
 
 class State(C.Structure):
@@ -97,7 +101,7 @@ def lib(variant: str = ""):
         L.orc_constants.argtypes = [_dp, C.c_int]
         M, S, A, F = (C.POINTER(t) for t in (Model, State, Aux, Forcing))
         L.orc_update_cache.argtypes = [M, S, F, A]
-        L.orc_update_implicit_cache.argtypes = [M, S, A]
+        L.orc_update_implicit_cache.argtypes = [M, S, F, A]
         L.orc_exp_tendency.argtypes = [M, S, A, S]
         L.orc_imp_tendency.argtypes = [M, S, A, S]
         L.orc_jacobian.argtypes = [M, S, A, d, _dp, _dp, _dp]
@@ -130,7 +134,8 @@ def _p(a):
 class OracleModel:
     """Oracle view of a CSR column batch (same arrays the library takes)."""
 
-    def __init__(self, model: int, bc_bottom: int, col_ptr, z_face, params, variant=""):
+    def __init__(self, model: int, bc_bottom: int, col_ptr, z_face, params, variant="",
+                 bc_top: int = BC_FLUX):
         self.L = lib(variant)
         self.col_ptr = np.ascontiguousarray(col_ptr, dtype=np.int32)
         self.z_face = np.ascontiguousarray(z_face, dtype=np.float64)
@@ -140,11 +145,12 @@ class OracleModel:
         self.ncol = self.col_ptr.size - 1
         self.ncell = int(self.col_ptr[-1])
         self._m = Model(model, bc_bottom, self.ncol, self.col_ptr.ctypes.data_as(_ip),
-                        _p(self.z_face), _p(self.params))
+                        _p(self.z_face), _p(self.params), bc_top, 0)
 
     @classmethod
     def from_batch(cls, b, variant=""):
-        return cls(b.model, b.bc_bottom, b.col_ptr, b.z_face, b.params, variant=variant)
+        return cls(b.model, b.bc_bottom, b.col_ptr, b.z_face, b.params, variant=variant,
+                   bc_top=getattr(b, "bc_top", BC_FLUX))
 
     def new_state(self, src: dict | None = None) -> dict:
         st = {"vartheta_l": np.zeros(self.ncell), "intF_water": np.zeros(self.ncol)}
@@ -179,8 +185,12 @@ class OracleModel:
         self.L.orc_update_cache(C.byref(self._m), C.byref(self._s(Y)), C.byref(f),
                                C.byref(self._a(aux)))
 
-    def update_implicit_cache(self, Y: dict, aux: dict):
+    def update_implicit_cache(self, Y: dict, aux: dict, tw=None, bw=None):
+        """cache_imp!; under a top MoistureStateBC also the boundary fluxes,
+        from the boundary states tw (top) and bw (bottom)."""
+        f = self._f(tw, None, bw, None)
         self.L.orc_update_implicit_cache(C.byref(self._m), C.byref(self._s(Y)),
+                                        C.byref(f) if tw is not None else None,
                                         C.byref(self._a(aux)))
 
     def exp_tendency(self, Y: dict, aux: dict) -> dict:

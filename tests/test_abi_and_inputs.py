@@ -62,6 +62,9 @@ def test_struct_layouts_match_header():
     (lambda c: setattr(c, "jac_precision", 16), -1),
     (lambda c: setattr(c, "newton_max_iters", 0), -1),
     (lambda c: setattr(c, "solver", 9), -1),
+    (lambda c: setattr(c, "bc_top", 1), -1),          # free drainage is a bottom condition
+    (lambda c: setattr(c, "bc_top", 7), -1),
+    (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 2)), -1),
 ])
 def test_create_rejects_bad_config_without_device(mutate, code):
     """Validation runs before any HIP call, so this is safe without a GPU."""
