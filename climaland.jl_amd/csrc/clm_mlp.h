@@ -347,6 +347,127 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(
     }
 }
 
+// ---- wide GEMM for the hidden layers: one block owns full output rows --------
+// C[M x N] = A[M x K] . B[N x K]^T with N = 128 NTW (384 or 256 here): a
+// 512-thread block computes 128 rows x N columns, so each row of A is read
+// from HBM once; waves are 2 (rows) x 4 (columns), each 64 x 32 NTW as
+// 2 x NTW MFMA 32x32x16 tiles.  K chunks of 32 alternate between two LDS
+// buffers: the next chunk's global loads are issued before the current
+// chunk's MFMAs and stored into the other buffer, one barrier per chunk.
+// Same epilogues and operand orientation as k_gemm_nt; K % 32 == 0.
+constexpr int kWBM = 128;
+
+template <int EPI, int NTW>
+__global__ __launch_bounds__(512) void k_gemm_wide(
+    int M, int K, const bf16* __restrict__ A, int lda, const bf16* __restrict__ B, int ldb,
+    const float* __restrict__ bias, float* __restrict__ Cf, int ldcf, bf16* __restrict__ Cb,
+    int ldcb, const float* __restrict__ Z, int ldz) {
+    constexpr int BN = 128 * NTW;
+    constexpr int NA = kWBM * kBK / 8 / 512;  // uint4 of A per thread per chunk (1)
+    constexpr int NB = BN * kBK / 8 / 512;    // uint4 of B per thread per chunk (NTW)
+    __shared__ __attribute__((aligned(16))) bf16 As[2][kWBM * kLdsRow];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN * kLdsRow];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 2, wn = w & 3;
+    const int m0 = blockIdx.x * kWBM;
+    floatx16 acc[2][NTW];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < NTW; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const uint4 zero4 = {0u, 0u, 0u, 0u};
+    uint4 ra[NA], rb[NB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < NA; ++h) {
+            int cidx = t + 512 * h;
+            int row = cidx >> 2, kq = (cidx & 3) * 8;
+            ra[h] = (m0 + row < M) ? *(const uint4*)(A + (int64_t)(m0 + row) * lda + k0 + kq)
+                                   : zero4;
+        }
+#pragma unroll
+        for (int h = 0; h < NB; ++h) {
+            int cidx = t + 512 * h;
+            int row = cidx >> 2, kq = (cidx & 3) * 8;
+            rb[h] = *(const uint4*)(B + (int64_t)row * ldb + k0 + kq);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < NA; ++h) {
+            int cidx = t + 512 * h;
+            *(uint4*)(&As[buf][(cidx >> 2) * kLdsRow + (cidx & 3) * 8]) = ra[h];
+        }
+#pragma unroll
+        for (int h = 0; h < NB; ++h) {
+            int cidx = t + 512 * h;
+            *(uint4*)(&Bs[buf][(cidx >> 2) * kLdsRow + (cidx & 3) * 8]) = rb[h];
+        }
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += kBK) {
+        const bool more = k0 + kBK < K;
+        if (more) gload(k0 + kBK);
+#pragma unroll
+        for (int ks = 0; ks < kBK; ks += 16) {
+            bf16x8 a[2], b[NTW];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *(const bf16x8*)(&As[buf][(wm * 64 + i * 32 + (lane & 31)) * kLdsRow + ks +
+                                                 (lane >> 5) * 8]);
+#pragma unroll
+            for (int j = 0; j < NTW; ++j)
+                b[j] = *(const bf16x8*)(&Bs[buf][(wn * 32 * NTW + j * 32 + (lane & 31)) * kLdsRow +
+                                                 ks + (lane >> 5) * 8]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < NTW; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0,
+                                                                        0, 0);
+        }
+        if (more) lstore(buf ^ 1);  // the other buffer was last read before the previous barrier
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int m = m0 + wm * 64 + i * 32 + (lane & 31);
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < NTW; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int n = wn * 32 * NTW + j * 32 + 8 * q + 4 * (lane >> 5);
+                float4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2],
+                            acc[i][j][4 * q + 3]};
+                if constexpr (EPI == EPI_BIAS_GELU) {
+                    float4 bb = *(const float4*)(bias + n);
+                    v.x += bb.x;
+                    v.y += bb.y;
+                    v.z += bb.z;
+                    v.w += bb.w;
+                    if (Cf) *(float4*)(Cf + (int64_t)m * ldcf + n) = v;
+                    bf16x4 h = {(bf16)gelu(v.x), (bf16)gelu(v.y), (bf16)gelu(v.z),
+                                (bf16)gelu(v.w)};
+                    *(bf16x4*)(Cb + (int64_t)m * ldcb + n) = h;
+                } else if constexpr (EPI == EPI_DGELU) {
+                    float4 z = *(const float4*)(Z + (int64_t)m * ldz + n);
+                    bf16x4 h = {(bf16)(v.x * dgelu(z.x)), (bf16)(v.y * dgelu(z.y)),
+                                (bf16)(v.z * dgelu(z.z)), (bf16)(v.w * dgelu(z.w))};
+                    *(bf16x4*)(Cb + (int64_t)m * ldcb + n) = h;
+                } else {
+                    *(float4*)(Cf + (int64_t)m * ldcf + n) = v;
+                }
+            }
+    }
+}
+
 // ---- small training kernels ---------------------------------------------------
 // out[c][r] = in[r][c] (bf16), 32 x 32 tiles through LDS
 __global__ __launch_bounds__(256) void k_transpose(int R, int Cn, const bf16* __restrict__ in,

@@ -551,6 +551,17 @@ template <int EPI>
 void ws_gemm(clm_ctx* c, int M, int N, int K, const bf16* A, int lda, const bf16* B, int ldb,
              const float* bias, float* Cf, int ldcf, bf16* Cb, int ldcb, const float* Z, int ldz,
              int S = 1) {
+    if (S == 1 && (EPI == ws::EPI_BIAS_GELU || EPI == ws::EPI_DGELU) && (N == 384 || N == 256)) {
+        // hidden-layer width: one block owns whole output rows
+        dim3 g((unsigned)((M + ws::kWBM - 1) / ws::kWBM));
+        if (N == 384)
+            hipLaunchKernelGGL((ws::k_gemm_wide<EPI, 3>), g, dim3(512), 0, c->stream, M, K, A,
+                               lda, B, ldb, bias, Cf, ldcf, Cb, ldcb, Z, ldz);
+        else
+            hipLaunchKernelGGL((ws::k_gemm_wide<EPI, 2>), g, dim3(512), 0, c->stream, M, K, A,
+                               lda, B, ldb, bias, Cf, ldcf, Cb, ldcb, Z, ldz);
+        return;
+    }
     dim3 g((unsigned)((M + ws::kBM - 1) / ws::kBM), (unsigned)((N + ws::kBN - 1) / ws::kBN),
            (unsigned)S);
     hipLaunchKernelGGL(ws::k_gemm_nt<EPI>, g, dim3(256), 0, c->stream, M, N, K, K / S, A, lda,
