@@ -464,6 +464,26 @@ class LandSimulation:
         self.t += self.Δt
         return st
 
+    def save_checkpoint(self, output_dir: str) -> str:
+        """ClimaLand.save_checkpoint(Y, t, output_dir): the device state and
+        time to <output_dir>/day<D>.<S>.clmchk; returns the file's path."""
+        m = self.model
+        self._push()
+        buf = C.create_string_buffer(4096)
+        m._check(m._lib.clm_save_checkpoint(m._ctx, output_dir.encode(), self.t, buf, 4096))
+        return buf.value.decode()
+
+    def restore_checkpoint(self, path: str) -> bool:
+        """ClimaLand.read_checkpoint: load state and time; returns whether the
+        file came from a model with the same parameters and grid."""
+        m = self.model
+        t = C.c_double(0.0)
+        same = C.c_int32(0)
+        m._check(m._lib.clm_read_checkpoint(m._ctx, path.encode(), C.byref(t), C.byref(same)))
+        self._on_device = True
+        self.t = t.value
+        return bool(same.value)
+
     def set_forcing_table(self, top_water, top_heat=None):
         """Upload a [slot][col] forcing schedule (DriverUpdateCallback values)."""
         m = self.model

@@ -38,6 +38,7 @@ struct clm_ctx {
     int64_t ncol = 0, ncell = 0;
     int nparam = 0;
     std::vector<int32_t> col_ptr;
+    std::vector<double> h_zf, h_par;  // host copies (checkpoint model hash)
     // layout
     int32_t npack = 0;
     int64_t nslot = 0, npcell = 0, npface = 0;
@@ -784,6 +785,8 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     c->nparam = need;
     c->col_ptr.assign(cfg->col_ptr, cfg->col_ptr + cfg->ncol + 1);
     c->ncell = c->col_ptr.back();
+    c->h_zf.assign(cfg->z_face, cfg->z_face + c->ncell + c->ncol);
+    c->h_par.assign(cfg->params, cfg->params + (size_t)need * c->ncol);
 
     // ---- layout: sort by layer count (descending), packs of 64 ----
     std::vector<int32_t> order((size_t)c->ncol);
@@ -1909,6 +1912,130 @@ int clm_ws_time(clm_ctx* c, double dt, int32_t reps, double* ms_per_call) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     *ms_per_call = ms / reps;
+    return CLM_OK;
+}
+
+
+// ---- checkpoints (checkpoints.jl:125-195) ----------------------------------------
+// File <dir>/day<D>.<S>.clmchk, D = floor(t / 86400 s), S = floor(t mod 86400 s).
+// Layout (little endian): char magic[8] = "CLMCHK01"; int32 model, bc_bottom,
+// bc_top, pad; int64 ncol, ncell; double t; uint64 model hash; then the state
+// as float64 arrays: vartheta_l[ncell], intF_water[ncol] and, for
+// EnergyHydrology, theta_i[ncell], rho_e_int[ncell], intF_energy[ncol].
+// The model hash (FNV-1a over the model kind, boundary kinds, col_ptr,
+// z_face and params) plays the part of the reference's land_model_hash.
+namespace {
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 0x100000001B3ULL;
+    }
+    return h;
+}
+struct ChkHeader {
+    char magic[8];
+    int32_t model, bc_bottom, bc_top, pad;
+    int64_t ncol, ncell;
+    double t;
+    uint64_t hash;
+};
+}  // namespace
+
+static uint64_t model_hash(const clm_ctx* c) {
+    uint64_t h = 0xCBF29CE484222325ULL;
+    int32_t kinds[3] = {c->model, c->bc_bottom, c->bc_top};
+    h = fnv1a(h, kinds, sizeof kinds);
+    h = fnv1a(h, c->col_ptr.data(), 4 * c->col_ptr.size());
+    h = fnv1a(h, c->h_zf.data(), 8 * c->h_zf.size());
+    h = fnv1a(h, c->h_par.data(), 8 * c->h_par.size());
+    return h;
+}
+
+int clm_save_checkpoint(clm_ctx* c, const char* dir, double t, char* path, int32_t path_len) {
+    if (!c || !dir || !(t >= 0.0)) return CLM_E_ARG;
+    const bool eh = is_eh(c);
+    std::vector<double> v((size_t)c->ncell), iw((size_t)c->ncol), ti, re, ie;
+    clm_state Y{v.data(), nullptr, nullptr, iw.data(), nullptr};
+    if (eh) {
+        ti.resize((size_t)c->ncell);
+        re.resize((size_t)c->ncell);
+        ie.resize((size_t)c->ncol);
+        Y.theta_i = ti.data();
+        Y.rho_e_int = re.data();
+        Y.intF_energy = ie.data();
+    }
+    int r = clm_get_state(c, &Y);
+    if (r) return r;
+    long long day = (long long)std::floor(t / 86400.0);
+    long long sec = (long long)std::floor(std::fmod(t, 86400.0));
+    std::string file = std::string(dir) + "/day" + std::to_string(day) + "." +
+                       std::to_string(sec) + ".clmchk";
+    FILE* f = std::fopen(file.c_str(), "wb");
+    if (!f) return set_err(c, CLM_E_ARG, "cannot open %s for writing", file.c_str());
+    ChkHeader h;
+    std::memcpy(h.magic, "CLMCHK01", 8);
+    h.model = c->model;
+    h.bc_bottom = c->bc_bottom;
+    h.bc_top = c->bc_top;
+    h.pad = 0;
+    h.ncol = c->ncol;
+    h.ncell = c->ncell;
+    h.t = t;
+    h.hash = model_hash(c);
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+    ok = ok && std::fwrite(v.data(), 8, v.size(), f) == v.size();
+    ok = ok && std::fwrite(iw.data(), 8, iw.size(), f) == iw.size();
+    if (eh) {
+        ok = ok && std::fwrite(ti.data(), 8, ti.size(), f) == ti.size();
+        ok = ok && std::fwrite(re.data(), 8, re.size(), f) == re.size();
+        ok = ok && std::fwrite(ie.data(), 8, ie.size(), f) == ie.size();
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) return set_err(c, CLM_E_ARG, "writing %s failed", file.c_str());
+    if (path && path_len > 0) std::snprintf(path, (size_t)path_len, "%s", file.c_str());
+    return CLM_OK;
+}
+
+int clm_read_checkpoint(clm_ctx* c, const char* path, double* t, int32_t* same_model) {
+    if (!c || !path) return CLM_E_ARG;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_err(c, CLM_E_ARG, "cannot open %s", path);
+    ChkHeader h;
+    bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, "CLMCHK01", 8) == 0;
+    if (!ok) {
+        std::fclose(f);
+        return set_err(c, CLM_E_ARG, "%s is not a checkpoint", path);
+    }
+    if (h.model != c->model || h.ncol != c->ncol || h.ncell != c->ncell) {
+        std::fclose(f);
+        return set_err(c, CLM_E_ARG, "%s holds a %s state of %lld columns / %lld cells", path,
+                       h.model ? "EnergyHydrology" : "RichardsModel", (long long)h.ncol,
+                       (long long)h.ncell);
+    }
+    const bool eh = is_eh(c);
+    std::vector<double> v((size_t)h.ncell), iw((size_t)h.ncol), ti, re, ie;
+    ok = std::fread(v.data(), 8, v.size(), f) == v.size();
+    ok = ok && std::fread(iw.data(), 8, iw.size(), f) == iw.size();
+    clm_state Y{v.data(), nullptr, nullptr, iw.data(), nullptr};
+    if (eh) {
+        ti.resize(v.size());
+        re.resize(v.size());
+        ie.resize(iw.size());
+        ok = ok && std::fread(ti.data(), 8, ti.size(), f) == ti.size();
+        ok = ok && std::fread(re.data(), 8, re.size(), f) == re.size();
+        ok = ok && std::fread(ie.data(), 8, ie.size(), f) == ie.size();
+        Y.theta_i = ti.data();
+        Y.rho_e_int = re.data();
+        Y.intF_energy = ie.data();
+    }
+    std::fclose(f);
+    if (!ok) return set_err(c, CLM_E_ARG, "%s is truncated", path);
+    int r = clm_set_state(c, &Y);
+    if (r) return r;
+    if (t) *t = h.t;
+    // the reference warns (and goes on) when the model differs
+    if (same_model) *same_model = h.hash == model_hash(c) ? 1 : 0;
     return CLM_OK;
 }
 

@@ -228,6 +228,20 @@ int clm_autotune(clm_ctx* ctx, double dt, int32_t reps);
  * launch (ms), measured with HIP events on the ctx stream. */
 int clm_last_kernel_ms(const clm_ctx* ctx, double* ms_per_launch, int32_t* launches);
 
+/* ---- checkpoints (save_checkpoint / read_checkpoint, checkpoints.jl:125-195) ----
+ * clm_save_checkpoint writes the device state and time t to
+ * <dir>/day<D>.<S>.clmchk (D = floor(t / 86400 s), S = floor(t mod 86400 s))
+ * and returns that path in `path` (may be NULL).  clm_read_checkpoint loads a
+ * checkpoint of the same model kind and column geometry into the device
+ * state and returns its time; *same_model = 0 where the file was written by
+ * a model with other parameters or grid (the reference warns and goes on).
+ * File layout: 64-byte header (magic "CLMCHK01", model, bc_bottom, bc_top,
+ * pad, ncol, ncell, t, model hash), then float64 vartheta_l[ncell],
+ * intF_water[ncol] and, for EnergyHydrology, theta_i[ncell],
+ * rho_e_int[ncell], intF_energy[ncol]. */
+int clm_save_checkpoint(clm_ctx* ctx, const char* dir, double t, char* path, int32_t path_len);
+int clm_read_checkpoint(clm_ctx* ctx, const char* path, double* t, int32_t* same_model);
+
 /* ---- learned Newton warm start (group solver) ---------------------------------
  * A per-column MLP predicts the implicit part of the step's increment,
  * x_{n+1} - U0, at 32 nodes (layer-index resampling); Newton then starts at
