@@ -278,6 +278,10 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);  // face below row
+    double idzf[R];  // reciprocal spacings (row 0's face is the boundary: unused)
+#pragma unroll
+    for (int k = 0; k < R; ++k) idzf[k] = dzf[k] != 0.0 ? 1.0 / dzf[k] : 0.0;
+    const Span sp = make_span(v, nu);
 
     // boundary fluxes from update_cache!(u_n); under a top MoistureStateBC
     // they follow the iterate (rre.jl:460-468) and the top row carries the
@@ -313,13 +317,13 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         for (int k = 0; k < R; ++k) x[k] = u0[k];
     }
     for (kit = 1; kit <= max_iters; ++kit) {
-        double K[R], h[R], D[R];
+        double K[R], h[R], iD[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            PsiD p = psi_dpsi(v, nu, x[k]);
+            PsiI p = psi_invd(v, sp, x[k]);
             K[k] = on[k] ? hyd_cond_from_a(v, p.S, p.a) : 0.0;
             h[k] = p.psi + zc[k];
-            D[k] = p.D;
+            iD[k] = p.invD;
         }
         if (top_state && gl == 0 && bc_bottom != BC_FLUX) {  // bottom flux at the iterate
             double psi0 = h[0] - zc[0];
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         for (int k = 0; k < R; ++k) {
             int r = gl * R + k;
             double Km = below<G, R>(K, k), hm = below<G, R>(h, k);
-            double ww = 0.5 * (Km + K[k]) / dzf[k];
+            double ww = 0.5 * (Km + K[k]) * idzf[k];
             w[k] = (r == 0) ? 0.0 : ww;
             F[k] = (r == 0) ? bot : -ww * (h[k] - hm);
         }
@@ -357,7 +361,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
             double rhs = dzc[k] * (u0[k] - x[k]) - dt * (Fu - F[k]);
             a[k] = (JT)(dt * w[k]);
             c[k] = (JT)(dt * wu);
-            b[k] = (JT)(-dt * (w[k] + wu + wbc) - dzc[k] / D[k]);
+            b[k] = (JT)(-dt * (w[k] + wu + wbc) - dzc[k] * iD[k]);
             d[k] = (JT)rhs;
             if (!on[k]) {
                 a[k] = 0;
@@ -370,7 +374,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         double e = 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            double dx = on[k] ? (double)y[k] / D[k] : 0.0;
+            double dx = on[k] ? (double)y[k] * iD[k] : 0.0;
             x[k] -= dx;
             e = nan_max(e, fabs(dx));
         }
@@ -417,7 +421,8 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     const double topw = ftw[col], toph = fth[col];
 
     double xv[R], xi[R], xe[R], u0v[R], u0e[R], zc[R], dzc[R], dzf[R];
-    double Kl[R], kl[R], rcs[R], Tp[R], Pp[R], D[R];
+    double Kl[R], kl[R], rcs[R], Tp[R], Pp[R], iD[R];
+    Span sp[R];  // saturation span of each row (θ_i is fixed for the step after stage 1)
     bool on[R];
     int bad = 0;
     // ---- stage 1: update_cache!(u_n) + phase change -> U0 -------------------
@@ -452,10 +457,14 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         rcs[k] = heat_capacity(a.thl, xi[k], e.rhoc_ds);  // 1/D of the ρe block (lagged θ_l)
         Tp[k] = a.T;
         Pp[k] = a.psi;
-        D[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).D;  // Wfact at x = U0
+        sp[k] = make_span(e.v, e.nu - xi[k]);
+        iD[k] = psi_invd(e.v, sp[k], xv[k]).invD;  // Wfact at x = U0
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);
+    double idzf[R];  // reciprocal spacings (row 0's face is the boundary: unused)
+#pragma unroll
+    for (int k = 0; k < R; ++k) idzf[k] = dzf[k] != 0.0 ? 1.0 / dzf[k] : 0.0;
     double botw, both;
     {
         double K0 = g_bcast<G>(Kl[0], 0), T0 = g_bcast<G>(Tp[0], 0);
@@ -467,14 +476,16 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             both = fbh[col];
         }
     }
-    // conductances of the face below each row (lagged K, κ)
-    double wK[R], wk[R];
+    // face means and conductances of the face below each row (lagged K, κ)
+    double Kf[R], kf[R], wK[R], wk[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         int r = gl * R + k;
         double Km = below<G, R>(Kl, k), km = below<G, R>(kl, k);
-        wK[k] = (r == 0) ? 0.0 : 0.5 * (Km + Kl[k]) / dzf[k];
-        wk[k] = (r == 0) ? 0.0 : 0.5 * (km + kl[k]) / dzf[k];
+        Kf[k] = 0.5 * (Km + Kl[k]);
+        kf[k] = 0.5 * (km + kl[k]);
+        wK[k] = (r == 0) ? 0.0 : Kf[k] * idzf[k];
+        wk[k] = (r == 0) ? 0.0 : kf[k] * idzf[k];
     }
 
     // the ρe block of W is fixed for the whole step (lagged κ and ρc_s): with
@@ -517,9 +528,9 @@ __global__ __launch_bounds__(256) void k_eh_coop(
                 xe[k] = u0e[k] + ge[base + r];
             }
             Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
-            PsiD pd = psi_dpsi(e.v, e.nu - xi[k], xv[k]);
+            PsiI pd = psi_invd(e.v, sp[k], xv[k]);
             Pp[k] = pd.psi;
-            D[k] = pd.D;
+            iD[k] = pd.invD;
         }
     }
     if (attempt == 1) {  // the warm start went non-finite: plain Newton from U0
@@ -539,7 +550,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             Pp[k] = a.psi;
             xv[k] = u0v[k];
             xe[k] = u0e[k];
-            D[k] = psi_dpsi(e.v, e.nu - xi[k], xv[k]).D;
+            iD[k] = psi_invd(e.v, sp[k], xv[k]).invD;
         }
     }
     for (kit = 1; kit <= max_iters; ++kit) {
@@ -555,11 +566,11 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         for (int k = 0; k < R; ++k) {
             int r = gl * R + k;
             double hm = below<G, R>(h, k), Tm = below<G, R>(Tp, k), Em = below<G, R>(E, k);
-            double gh = (h[k] - hm) / dzf[k];
+            double gh = (h[k] - hm) * idzf[k];
             double eK = 0.5 * (Em + E[k]);
-            wE[k] = (r == 0) ? 0.0 : eK / dzf[k];
-            F[k] = (r == 0) ? botw : -(wK[k] * dzf[k]) * gh;
-            Gf[k] = (r == 0) ? both : -(wk[k] * dzf[k]) * ((Tp[k] - Tm) / dzf[k]) - eK * gh;
+            wE[k] = (r == 0) ? 0.0 : eK * idzf[k];
+            F[k] = (r == 0) ? botw : -Kf[k] * gh;
+            Gf[k] = (r == 0) ? both : -kf[k] * ((Tp[k] - Tm) * idzf[k]) - eK * gh;
         }
         JT a[R], b[R], c[R], d[R], y[R];
         double dRe[R];  // Δz_c R_ρe
@@ -581,7 +592,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             dRe[k] = dzc[k] * (u0e[k] - xe[k]) - dt * (Gu - Gf[k]);
             a[k] = (JT)(dt * wK[k]);
             c[k] = (JT)(dt * wu);
-            b[k] = (JT)(-dt * (wK[k] + wu) - dzc[k] / D[k]);
+            b[k] = (JT)(-dt * (wK[k] + wu) - dzc[k] * iD[k]);
             d[k] = (JT)rv;
             if (!on[k]) {
                 a[k] = 0;
@@ -597,7 +608,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             yv[k] = on[k] ? (double)y[k] : 0.0;
-            dv[k] = on[k] ? yv[k] / D[k] : 0.0;
+            dv[k] = on[k] ? yv[k] * iD[k] : 0.0;
         }
         double e2 = 0.0;
 #pragma unroll
@@ -639,9 +650,9 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             // cache_imp!(x): T and ψ at the new iterate (energy_hydrology.jl:427-445);
             // dψ/dϑ of the same evaluation is the next Wfact(x)
             Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
-            PsiD pd = psi_dpsi(e.v, e.nu - xi[k], xv[k]);
+            PsiI pd = psi_invd(e.v, sp[k], xv[k]);
             Pp[k] = pd.psi;
-            D[k] = pd.D;
+            iD[k] = pd.invD;
         }
         err = g_max<G>(e2);
         conv = (tol > 0.0) && (err < tol);

@@ -118,6 +118,48 @@ __device__ __forceinline__ PsiD psi_dpsi(const VG& v, double nu_eff, double vl) 
     return r;
 }
 
+// The same closures for the Newton loops of the fused kernels: the
+// saturation span is fixed per cell during a step (ν_eff moves only with θ_i,
+// which is explicit), so its reciprocal comes precomputed, and the Jacobian
+// needs dψ/dϑ only through its reciprocal 1/D (scaled unknowns and the
+// update Δ = y / D), which one division gives: 1/D = span X S / (cD X^(1/n) a).
+struct PsiI {
+    double psi, invD, S, a;
+};
+
+struct Span {
+    double ns, inv;  // max(ν_eff, θ_r + sqrt(eps)) and 1 / (that - θ_r)
+};
+__device__ __forceinline__ Span make_span(const VG& v, double nu_eff) {
+    Span sp;
+    sp.ns = fmax(nu_eff, v.thr_safe);
+    sp.inv = 1.0 / (sp.ns - v.theta_r);
+    return sp;
+}
+
+__device__ __forceinline__ PsiI psi_invd(const VG& v, const Span& sp, double vl) {
+    PsiI r;
+    double vs = fmax(vl, v.thr_safe);
+    double S = (vs - v.theta_r) * sp.inv;
+    r.S = S;
+    if (S < 1.0) {
+        double lS = log(S);
+        double a = exp(-lS * v.inv_m);
+        double X = a - 1.0;
+        double lX = log(X);
+        double X1n = exp(lX * v.inv_n);
+        r.a = a;
+        r.psi = -X1n * v.inv_alpha;
+        // D is +Inf where X == 0 (within an ulp of S = 1): 1/D = 0
+        r.invD = (X > 0.0) ? (X * S) / (v.cD * sp.inv * X1n * a) : 0.0;
+    } else {
+        r.a = 1.0;
+        r.psi = (S == 1.0) ? -0.0 : (vs - sp.ns) * v.inv_Ss;
+        r.invD = v.Ss;
+    }
+    return r;
+}
+
 // hydraulic_conductivity (:161-173) given S and a = S^(-1/m)
 __device__ __forceinline__ double hyd_cond_from_a(const VG& v, double S, double a) {
     if (S < 1.0) {
