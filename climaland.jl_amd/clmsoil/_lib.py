@@ -16,11 +16,13 @@ LIB_PATH = os.environ.get("CLMSOIL_LIB", os.path.join(PKG_ROOT, "libclmsoil.so")
 # mirrors of the enums in include/clm_soil.h
 CLM_OK, CLM_E_ARG, CLM_E_HIP, CLM_E_DOMAIN, CLM_E_STATE = 0, -1, -2, -3, -4
 MODEL_RICHARDS, MODEL_ENERGY_HYDROLOGY = 0, 1
-BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE = 0, 1, 2
+BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE, BC_TOPMODEL, BC_SURFACE_RUNOFF = 0, 1, 2, 3, 4
 SOLVER_AUTO, SOLVER_GROUP, SOLVER_LANE = 0, 1, 2
 (P_NU, P_THETA_R, P_VG_ALPHA, P_VG_N, P_KSAT, P_SS, P_NU_SS_OM, P_NU_SS_QUARTZ,
  P_NU_SS_GRAVEL, P_KAPPA_DRY, P_KAPPA_SAT_FROZEN, P_KAPPA_SAT_UNFROZEN, P_RHOC_DS) = range(13)
 NPARAM_RRE, NPARAM_EH = 6, 13
+P_F_MAX, P_F_OVER, P_R_SB = 6, 7, 8
+NPARAM_RRE_TOPMODEL = 9
 
 EXPORTED = (
     "clm_abi_version", "clm_create", "clm_destroy", "clm_last_error", "clm_set_state",
@@ -55,7 +57,7 @@ class State(C.Structure):
 class Aux(C.Structure):
     _fields_ = [("K", _dp), ("psi", _dp), ("theta_l", _dp), ("T", _dp), ("kappa", _dp),
                 ("top_water", _dp), ("top_heat", _dp), ("bottom_water", _dp),
-                ("bottom_heat", _dp)]
+                ("bottom_heat", _dp), ("sink", _dp), ("r_ss", _dp)]
 
 
 class StepStats(C.Structure):

@@ -200,6 +200,50 @@ class ColumnBatch:
         return b
 
 
+    def with_runoff(self, kind: int, seed: int = 0) -> "ColumnBatch":
+        """The same RichardsModel columns under RichardsAtmosDrivenFluxBC with
+        TOPMODELRunoff (kind L.BC_TOPMODEL) or SurfaceRunoff (L.BC_SURFACE_RUNOFF).
+
+        The top forcing becomes the precipitation P = -K_sat U[0.1, 3] per
+        column and slot, so that both branches of max(-K_sat, P) occur.  A
+        water table at height f·depth above the bottom, f ~ U[0.1, 0.6], makes
+        the cells below it saturated and hydrostatic, ϑ = ν + S_s (z_wt - z);
+        a fifth of the columns have no water table (f = 0) and, under
+        SurfaceRunoff, another fifth are saturated to the top (f = 1).
+        TOPMODEL rows: f_max ~ U[0.2, 0.6], f_over = 3.28 1/m and
+        R_sb = 1.484e-7 U[0.5, 2] m/s (the values of the reference's runoff
+        test, test/standalone/Soil/runoff.jl:101-103)."""
+        assert self.model == L.MODEL_RICHARDS
+        assert kind in (L.BC_TOPMODEL, L.BC_SURFACE_RUNOFF)
+        rng = np.random.default_rng(seed)
+        P = self.params
+        nc = self.ncol
+        precip = -P[L.P_KSAT][None, :] * rng.uniform(0.1, 3.0, self.top_water.shape)
+        f = rng.uniform(0.1, 0.6, nc)
+        pick = rng.random(nc)
+        f[pick < 0.2] = 0.0
+        if kind == L.BC_SURFACE_RUNOFF:
+            f[pick > 0.8] = 1.0
+        nl = self.nlayers
+        ccol = np.repeat(np.arange(nc), nl)
+        fb = np.arange(self.ncell) + ccol
+        zb, zt = self.z_face[self.col_ptr[:-1] + np.arange(nc)], self.z_face[self.col_ptr[1:] +
+                                                                           np.arange(nc)]
+        zwt = zb + f * (zt - zb)
+        zc = 0.5 * (self.z_face[fb] + self.z_face[fb + 1])
+        nu, ss = P[L.P_NU][ccol], P[L.P_SS][ccol]
+        below = zc < zwt[ccol]
+        vl = np.where(below, nu + ss * (zwt[ccol] - zc), np.minimum(self.vartheta_l, 0.98 * nu))
+        if kind == L.BC_TOPMODEL:
+            extra = np.stack([rng.uniform(0.2, 0.6, nc), np.full(nc, 3.28),
+                              1.484e-7 * rng.uniform(0.5, 2.0, nc)])
+            params = np.ascontiguousarray(np.concatenate([P[:L.NPARAM_RRE], extra]))
+        else:
+            params = P
+        return dataclasses.replace(self, params=params, vartheta_l=vl,
+                                   top_water=np.ascontiguousarray(precip), bc_top=kind)
+
+
 def kappa_solid(nu_om, nu_q):
     return KAPPA_OM ** nu_om * KAPPA_QUARTZ ** nu_q * KAPPA_MINERALS ** (1.0 - nu_om - nu_q)
 

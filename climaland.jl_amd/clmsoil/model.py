@@ -123,6 +123,34 @@ class FreeDrainage:
     pass
 
 
+class NoRunoff:
+    """Runoff.jl:52-75: infiltration = precipitation."""
+
+
+class SurfaceRunoff:
+    """Runoff.jl:84-148: no infiltration into a saturated top layer, else
+    max(-K_sat, P)."""
+
+
+@dataclasses.dataclass
+class TOPMODELRunoff:
+    """Runoff.jl:199-271 (Niu et al. 2005): surface infiltration from the
+    saturated fraction and the TOPMODELSubsurfaceRunoff sink (R_sb, f_over).
+    Each field is a scalar or one value per column."""
+    f_over: float | np.ndarray
+    f_max: float | np.ndarray
+    R_sb: float | np.ndarray
+
+
+@dataclasses.dataclass
+class RichardsAtmosDrivenFluxBC:
+    """boundary_conditions.jl:79-212: the top flux of a RichardsModel driven
+    by the liquid precipitation rate `precip` (per column, m/s, negative
+    downward) through a runoff model."""
+    precip: np.ndarray
+    runoff: object = dataclasses.field(default_factory=NoRunoff)
+
+
 class EnergyWaterFreeDrainage:
     pass
 
@@ -194,6 +222,8 @@ class _SoilModel:
         top = self.boundary_conditions["top"]
         if isinstance(top, WaterHeatBC):
             return np.asarray(top.water.bc, float), np.asarray(top.heat.bc, float)
+        if isinstance(top, RichardsAtmosDrivenFluxBC):
+            return np.asarray(top.precip, float), None
         return np.asarray(top.bc, float), None
 
     def _bottom_kind(self):
@@ -205,6 +235,12 @@ class _SoilModel:
 
     def _top_kind(self):
         top = self.boundary_conditions["top"]
+        if isinstance(top, RichardsAtmosDrivenFluxBC):
+            if isinstance(top.runoff, TOPMODELRunoff):
+                return L.BC_TOPMODEL
+            if isinstance(top.runoff, SurfaceRunoff):
+                return L.BC_SURFACE_RUNOFF
+            return L.BC_FLUX  # NoRunoff: the precipitation is the flux
         return L.BC_MOISTURE_STATE if isinstance(top, MoistureStateBC) else L.BC_FLUX
 
     def set_forcing(self):
@@ -224,15 +260,19 @@ class _SoilModel:
                                               L.dptr(bh)))
 
     def aux(self) -> dict:
-        """Copy of the cache p.soil (K, ψ, [θ_l, T, κ], boundary fluxes)."""
+        """Copy of the cache p.soil (K, ψ, [θ_l, T, κ], boundary fluxes,
+        [TOPMODEL: subsurface sink per cell and R_ss])."""
         d = self.domain
         a = {"K": np.zeros(d.ncell), "psi": np.zeros(d.ncell), "top_water": np.zeros(d.ncol),
              "bottom_water": np.zeros(d.ncol)}
         if self.is_eh:
             a.update(theta_l=np.zeros(d.ncell), T=np.zeros(d.ncell), kappa=np.zeros(d.ncell),
                      top_heat=np.zeros(d.ncol), bottom_heat=np.zeros(d.ncol))
+        if self._top_kind() == L.BC_TOPMODEL:
+            a.update(sink=np.zeros(d.ncell), r_ss=np.zeros(d.ncol))
         s = L.Aux(*(L.dptr(a.get(k)) for k in ("K", "psi", "theta_l", "T", "kappa", "top_water",
-                                               "top_heat", "bottom_water", "bottom_heat")))
+                                               "top_heat", "bottom_water", "bottom_heat",
+                                               "sink", "r_ss")))
         self._check(self._lib.clm_get_aux(self._ctx, C.byref(s)))
         return a
 
@@ -269,6 +309,9 @@ class RichardsModel(_SoilModel):
         p = self.parameters
         n = self.domain.ncol
         rows = [p.ν, p.θ_r, p.hydrology_cm.α, p.hydrology_cm.n, p.K_sat, p.S_s]
+        top = self.boundary_conditions["top"]
+        if isinstance(top, RichardsAtmosDrivenFluxBC) and isinstance(top.runoff, TOPMODELRunoff):
+            rows += [top.runoff.f_max, top.runoff.f_over, top.runoff.R_sb]
         return np.stack([np.broadcast_to(np.asarray(r, float), (n,)) for r in rows])
 
 
@@ -574,8 +617,16 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
               newton_tol=w.newton_tol if tol is None else tol, solver=solver)
     if batch.model == L.MODEL_RICHARDS:
         par = RichardsParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R])
-        top = MoistureStateBC(batch.top_water[slot]) if batch.bc_top == L.BC_MOISTURE_STATE \
-            else WaterFluxBC(batch.top_water[slot])
+        tw = batch.top_water[slot]
+        if batch.bc_top == L.BC_MOISTURE_STATE:
+            top = MoistureStateBC(tw)
+        elif batch.bc_top == L.BC_TOPMODEL:
+            top = RichardsAtmosDrivenFluxBC(tw, TOPMODELRunoff(
+                f_over=P[L.P_F_OVER], f_max=P[L.P_F_MAX], R_sb=P[L.P_R_SB]))
+        elif batch.bc_top == L.BC_SURFACE_RUNOFF:
+            top = RichardsAtmosDrivenFluxBC(tw, SurfaceRunoff())
+        else:
+            top = WaterFluxBC(tw)
         bottom = {L.BC_FREE_DRAINAGE: FreeDrainage(),
                   L.BC_MOISTURE_STATE: MoistureStateBC(batch.bottom_water),
                   L.BC_FLUX: WaterFluxBC(batch.bottom_water)}[batch.bc_bottom]
@@ -594,7 +645,8 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
 __all__ = [
     "vanGenuchten", "RichardsParameters", "EnergyHydrologyParameters", "ColumnBatchDomain",
     "WaterFluxBC", "HeatFluxBC", "FreeDrainage", "EnergyWaterFreeDrainage", "WaterHeatBC",
-    "MoistureStateBC", "NewtonWarmStart",
+    "MoistureStateBC", "NewtonWarmStart", "RichardsAtmosDrivenFluxBC", "NoRunoff",
+    "SurfaceRunoff", "TOPMODELRunoff",
     "RichardsModel", "EnergyHydrology", "make_update_cache", "make_update_implicit_cache",
     "make_exp_tendency", "make_compute_imp_tendency", "make_compute_jacobian",
     "initialize_jacobian", "LandSimulation", "model_from_batch",

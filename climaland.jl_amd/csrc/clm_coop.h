@@ -39,6 +39,12 @@ __device__ __forceinline__ T g_bcast(T v, int src) {
     return __shfl(v, src, G);
 }
 template <int G>
+__device__ __forceinline__ double g_sum(double v) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+template <int G>
 __device__ __forceinline__ double g_max(double v) {
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, G));
@@ -256,7 +262,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
                          par[P_THETA_R * ncol + col], par[P_SS * ncol + col],
                          par[P_KSAT * ncol + col]);
     const double nu = par[P_NU * ncol + col];
-    const double top = f_top[col];
+    double top = f_top[col];
 
     double x[R], u0[R], zc[R], dzc[R], dzf[R];
     bool on[R];
@@ -282,6 +288,30 @@ __global__ __launch_bounds__(256) void k_rre_coop(
 #pragma unroll
     for (int k = 0; k < R; ++k) idzf[k] = dzf[k] != 0.0 ? 1.0 / dzf[k] : 0.0;
     const Span sp = make_span(v, nu);
+
+    // runoff from update_cache!(u_n), held for the step (Runoff.jl:234-271):
+    // TOPMODEL's infiltration and subsurface sink, or SurfaceRunoff's infiltration
+    const bool topm = bc_top(bc) == BC_TOPMODEL;
+    double sink[R], rss = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) sink[k] = 0.0;
+    if (topm) {
+        const double thr = par[P_THETA_R * ncol + col];
+        double hp = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            sink[k] = on[k] ? sat_weight(u0[k], nu, thr) : 0.0;
+            hp += sink[k] * dzc[k];
+        }
+        Runoff o = topmodel(g_sum<G>(hp), zf[fbase + N] - zf[fbase], par[P_F_MAX * ncol + col],
+                            par[P_F_OVER * ncol + col], par[P_R_SB * ncol + col], v.Ksat, top);
+        top = o.infil;
+        rss = o.rss;
+#pragma unroll
+        for (int k = 0; k < R; ++k) sink[k] *= o.scale;
+    } else if (bc_top(bc) == BC_SURFACE_RUNOFF) {
+        top = surface_infiltration(xg[base + N - 1], nu, v.Ksat, top);
+    }
 
     // boundary fluxes from update_cache!(u_n); under a top MoistureStateBC
     // they follow the iterate (rre.jl:460-468) and the top row carries the
@@ -357,8 +387,9 @@ __global__ __launch_bounds__(256) void k_rre_coop(
                     top_used = Fu;
                 }
             }
-            // Δz_c R = Δz_c (U0 - x) - Δt (F_up - F_down)
+            // Δz_c R = Δz_c (U0 - x) - Δt (F_up - F_down) - Δt Δz_c sink
             double rhs = dzc[k] * (u0[k] - x[k]) - dt * (Fu - F[k]);
+            if (topm) rhs -= dt * (dzc[k] * sink[k]);
             a[k] = (JT)(dt * w[k]);
             c[k] = (JT)(dt * wu);
             b[k] = (JT)(-dt * (w[k] + wu + wbc) - dzc[k] * iD[k]);
@@ -390,7 +421,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         if (on[k]) xg[base + gl * R + k] = x[k];
     if (top_state) top_used = g_bcast<G>(top_used, (N - 1) / R);
     if (gl == 0) {
-        iw[col] = iw[col] + dt * (-(top_used - bot));  // ∫F: -I block, fluxes of the last iteration
+        iw[col] = iw[col] + dt * (-(top_used - bot) - rss);  // ∫F: -I block, fluxes of the last iteration
         int32_t flag = kit_total;
         if (tol > 0.0 && !conv) flag |= 0x10000;
         if (!isfinite(err)) flag |= 0x20000;

@@ -14,7 +14,11 @@
 
 namespace clm {
 
-enum { BC_FLUX = 0, BC_FREE_DRAINAGE = 1, BC_MOISTURE_STATE = 2 };
+// BC_TOPMODEL and BC_SURFACE_RUNOFF are top kinds of RichardsModel:
+// RichardsAtmosDrivenFluxBC(precipitation, runoff) (boundary_conditions.jl:79-212)
+// with TOPMODELRunoff or SurfaceRunoff; the forcing value is the precipitation
+enum { BC_FLUX = 0, BC_FREE_DRAINAGE = 1, BC_MOISTURE_STATE = 2, BC_TOPMODEL = 3,
+       BC_SURFACE_RUNOFF = 4 };
 // RichardsModel kernels take both boundary kinds packed as bottom | top << 4
 __host__ __device__ inline int bc_bot(int bc) { return bc & 15; }
 __host__ __device__ inline int bc_top(int bc) { return bc >> 4; }
@@ -34,12 +38,47 @@ __device__ __forceinline__ double moisture_flux_bot(const VG& v, double nu, doub
     return -K_bot * ((psi_bot + dz) - psib) / dz;
 }
 
+// ---- runoff (Runoff/Runoff.jl) -------------------------------------------------
+// saturation weight of a layer, is_saturated(ϑ_l - θ_r, ν - θ_r) (ϑ_l - θ_r)/(ν - θ_r)
+// (Runoff.jl:261-262, 438-440; heaviside(x) = x > eps, utils.jl:93-99)
+__device__ __forceinline__ double sat_weight(double vl, double nu, double thr) {
+    double twc = vl - thr, nsat = nu - thr;
+    return (twc - nsat > DEPS) ? twc / nsat : 0.0;
+}
+struct Runoff {
+    double infil;  // top boundary flux (infiltration)
+    double rss;    // subsurface runoff R_ss (m/s)
+    double scale;  // sink of a layer = scale · saturation weight
+};
+// TOPMODEL (Runoff.jl:234-271, 373-376, 386-389, 427-429; Niu et al. 2005
+// eqs 8, 11, 12) from h∇ = Σ weight Δz_c: z∇ = depth - h∇,
+// f_sat = min(f_max e^{-f_over z∇/2}, 1), infiltration = (1 - f_sat) max(-K_sat, P),
+// R_ss = R_sb e^{-f_over z∇}; the source (Runoff.jl:321-332) removes
+// R_ss / max(h∇, eps) · weight from each layer and R_ss from ∫F
+__device__ __forceinline__ Runoff topmodel(double h, double depth, double f_max, double f_over,
+                                           double R_sb, double Ksat, double precip) {
+    Runoff o;
+    double zw = depth - h;
+    double fsat = fmin(f_max * exp(-f_over / 2.0 * zw), 1.0);
+    o.infil = (1.0 - fsat) * fmax(-1.0 * Ksat, precip);
+    o.rss = R_sb * exp(-f_over * zw);
+    o.scale = o.rss / fmax(h, DEPS);
+    return o;
+}
+// SurfaceRunoff (Runoff.jl:107-148): (1 - is_saturated(ϑ_top, ν)) max(-K_sat, P)
+__device__ __forceinline__ double surface_infiltration(double vl_top, double nu, double Ksat,
+                                                       double precip) {
+    return (vl_top - nu > DEPS) ? 0.0 : fmax(-1.0 * Ksat, precip);
+}
+
 // per-slot parameter rows (same order as CLM_P_* in include/clm_soil.h)
 enum {
     P_NU = 0, P_THETA_R, P_VG_ALPHA, P_VG_N, P_KSAT, P_SS,
     P_NU_SS_OM, P_NU_SS_QUARTZ, P_NU_SS_GRAVEL, P_KAPPA_DRY,
     P_KAPPA_SAT_FROZEN, P_KAPPA_SAT_UNFROZEN, P_RHOC_DS, NPARAM_EH
 };
+// RichardsModel under a TOPMODEL top boundary: rows 6-8
+enum { P_F_MAX = 6, P_F_OVER = 7, P_R_SB = 8, NPARAM_RRE_TOPMODEL = 9 };
 
 __device__ __forceinline__ bool lane_column(const Layout& L, ColView& cv) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,15 +108,19 @@ __global__ __launch_bounds__(256) void k_rre_update_cache(
     Layout L, const double* __restrict__ par, const double* __restrict__ zf,
     const double* __restrict__ vl, double* __restrict__ K, double* __restrict__ psi,
     const double* __restrict__ f_top, const double* __restrict__ f_bot, double* __restrict__ tw,
-    double* __restrict__ bw, int bc, int do_bc) {
+    double* __restrict__ bw, int bc, int do_bc, double* __restrict__ sink,
+    double* __restrict__ rss) {
     ColView cv;
     if (!lane_column(L, cv)) return;
     VG v = load_vg(par, L.nslot, cv.s);
     double nu = par[P_NU * L.nslot + cv.s];
-    double K0 = 0.0, p0 = 0.0, KN = 0.0, pN = 0.0;
+    const double thr = par[P_THETA_R * L.nslot + cv.s];
+    const bool topm = do_bc && bc_top(bc) == BC_TOPMODEL;
+    double K0 = 0.0, p0 = 0.0, KN = 0.0, pN = 0.0, xN = 0.0, h = 0.0;
     for (int j = 0; j < cv.N; ++j) {
         int64_t c = cv.cb + (int64_t)kWave * j;
-        PsiD r = psi_dpsi(v, nu, vl[c]);
+        double x = vl[c];
+        PsiD r = psi_dpsi(v, nu, x);
         double k = hyd_cond_from_a(v, r.S, r.a);
         K[c] = k;
         psi[c] = r.psi;
@@ -87,10 +130,26 @@ __global__ __launch_bounds__(256) void k_rre_update_cache(
         }
         KN = k;
         pN = r.psi;
+        xN = x;
+        if (topm) {
+            double wgt = sat_weight(x, nu, thr);
+            sink[c] = wgt;
+            h += wgt * (zf[cv.fb + (int64_t)kWave * (j + 1)] - zf[cv.fb + (int64_t)kWave * j]);
+        }
     }
     if (do_bc) {
         const int N = cv.N;
-        if (bc_top(bc) == BC_MOISTURE_STATE) {
+        if (topm) {
+            const int64_t ns = L.nslot;
+            Runoff o = topmodel(h, zf[cv.fb + (int64_t)kWave * N] - zf[cv.fb],
+                                par[P_F_MAX * ns + cv.s], par[P_F_OVER * ns + cv.s],
+                                par[P_R_SB * ns + cv.s], v.Ksat, f_top[cv.s]);
+            tw[cv.s] = o.infil;
+            rss[cv.s] = o.rss;
+            for (int j = 0; j < N; ++j) sink[cv.cb + (int64_t)kWave * j] *= o.scale;
+        } else if (bc_top(bc) == BC_SURFACE_RUNOFF) {
+            tw[cv.s] = surface_infiltration(xN, nu, v.Ksat, f_top[cv.s]);
+        } else if (bc_top(bc) == BC_MOISTURE_STATE) {
             double dz = 0.5 * (zf[cv.fb + (int64_t)kWave * N] - zf[cv.fb + (int64_t)kWave * (N - 1)]);
             tw[cv.s] = moisture_flux_top(v, nu, f_top[cv.s], KN, pN, dz);
         } else {
@@ -111,12 +170,14 @@ __global__ __launch_bounds__(256) void k_rre_update_cache(
 __global__ __launch_bounds__(256) void k_rre_imp_tendency(
     Layout L, const double* __restrict__ zf, const double* __restrict__ K,
     const double* __restrict__ psi, const double* __restrict__ tw,
-    const double* __restrict__ bw, double* __restrict__ dvl, double* __restrict__ diw) {
+    const double* __restrict__ bw, double* __restrict__ dvl, double* __restrict__ diw,
+    const double* __restrict__ sink, const double* __restrict__ rss) {
+    // sink / rss: TOPMODEL subsurface runoff (implicit source, Runoff.jl:321-332), or null
     ColView cv;
     if (!lane_column(L, cv)) return;
     const int N = cv.N;
     double top = tw[cv.s], bot = bw[cv.s];
-    diw[cv.s] = -(top - bot);
+    diw[cv.s] = sink ? -(top - bot) - rss[cv.s] : -(top - bot);
     double f0 = zf[cv.fb], f1 = zf[cv.fb + kWave];
     double zc_prev = 0.5 * (f0 + f1);
     double Flo = bot;
@@ -138,7 +199,9 @@ __global__ __launch_bounds__(256) void k_rre_imp_tendency(
             K_prev = Kn;
             h_prev = hn;
         }
-        dvl[cv.cb + (int64_t)kWave * i] = -(Fhi - Flo) / dzc_i;
+        double t = -(Fhi - Flo) / dzc_i;
+        if (sink) t -= sink[cv.cb + (int64_t)kWave * i];
+        dvl[cv.cb + (int64_t)kWave * i] = t;
         Flo = Fhi;
     }
 }
@@ -261,10 +324,28 @@ __global__ __launch_bounds__(256) void k_rre_step(
     const double nu = par[P_NU * L.nslot + cv.s];
     const int bc_bottom = bc_bot(bc);
     const bool top_state = bc_top(bc) == BC_MOISTURE_STATE;
+    const bool topm = bc_top(bc) == BC_TOPMODEL;
     const double ftop = f_top[cv.s];
     double top = ftop;
     double bot = (bc_bottom == BC_FLUX) ? f_bot[cv.s] : 0.0;
     const double iw0 = iw[cv.s];
+    const double thr = par[P_THETA_R * L.nslot + cv.s];
+    // runoff from update_cache!(u_n), held for the step (Runoff.jl:234-271)
+    double sscale = 0.0, rss = 0.0;
+    if (topm) {
+        double h = 0.0;
+        for (int i = 0; i < N; ++i)
+            h += sat_weight(x[cb + (int64_t)kWave * i], nu, thr) *
+                 (zf[fb + (int64_t)kWave * (i + 1)] - zf[fb + (int64_t)kWave * i]);
+        const int64_t ns = L.nslot;
+        Runoff o = topmodel(h, zf[fb + (int64_t)kWave * N] - zf[fb], par[P_F_MAX * ns + cv.s],
+                            par[P_F_OVER * ns + cv.s], par[P_R_SB * ns + cv.s], v.Ksat, ftop);
+        top = o.infil;
+        rss = o.rss;
+        sscale = o.scale;
+    } else if (bc_top(bc) == BC_SURFACE_RUNOFF) {
+        top = surface_infiltration(x[cb + (int64_t)kWave * (N - 1)], nu, v.Ksat, ftop);
+    }
 
     int k = 0;
     bool conv = false;
@@ -311,6 +392,7 @@ __global__ __launch_bounds__(256) void k_rre_step(
             double u0c = u0[c];
             if (k == 1) U0[c] = u0c;
             double R = u0c + dt * (-(Fhi - Flo) * idzc) - x_c;
+            if (topm) R -= dt * (sscale * sat_weight(u0c, nu, thr));
             double s = dt * idzc;
             double wbc = (i == N - 1 && top_state) ? K_c / (0.5 * dzc_c) : 0.0;
             JT lo = (JT)(s * wlo * D_m);
@@ -342,7 +424,7 @@ __global__ __launch_bounds__(256) void k_rre_step(
             x[c] = x[c] - d;
             err = nan_max(err, fabs(d));
         }
-        iw[cv.s] = iw0 + dt * (-(top - bot));  // ∫F: -I block, fluxes of this iteration
+        iw[cv.s] = iw0 + dt * (-(top - bot) - rss);  // ∫F: -I block, fluxes of this iteration
         conv = (tol > 0.0) && (err < tol);
         // a non-finite iterate stays non-finite: stop (the state is NaN either way)
         if (conv || !isfinite(err)) break;

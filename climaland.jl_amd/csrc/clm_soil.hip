@@ -59,6 +59,7 @@ struct clm_ctx {
     // aux (fine hooks)
     double *d_K = nullptr, *d_psi = nullptr, *d_thl = nullptr, *d_T = nullptr, *d_kap = nullptr;
     double *d_tw = nullptr, *d_th = nullptr, *d_bw = nullptr, *d_bh = nullptr;
+    double *d_sink = nullptr, *d_rss = nullptr;  // TOPMODEL subsurface runoff
     bool have_cache = false, have_jac = false;
     // fine-hook work buffers
     double* d_wy[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -733,12 +734,19 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     if (cfg->bc_bottom != CLM_BC_FLUX && cfg->bc_bottom != CLM_BC_FREE_DRAINAGE &&
         cfg->bc_bottom != CLM_BC_MOISTURE_STATE)
         return set_err(nullptr, CLM_E_ARG, "unknown bottom boundary %d", cfg->bc_bottom);
-    if (cfg->bc_top != CLM_BC_FLUX && cfg->bc_top != CLM_BC_MOISTURE_STATE)
+    if (cfg->bc_top != CLM_BC_FLUX && cfg->bc_top != CLM_BC_MOISTURE_STATE &&
+        cfg->bc_top != CLM_BC_TOPMODEL && cfg->bc_top != CLM_BC_SURFACE_RUNOFF)
         return set_err(nullptr, CLM_E_ARG, "unknown top boundary %d", cfg->bc_top);
     if (cfg->model == CLM_MODEL_ENERGY_HYDROLOGY &&
         (cfg->bc_top == CLM_BC_MOISTURE_STATE || cfg->bc_bottom == CLM_BC_MOISTURE_STATE))
         return set_err(nullptr, CLM_E_ARG, "MoistureStateBC is implemented for RichardsModel only");
-    int need = cfg->model == CLM_MODEL_RICHARDS ? CLM_NPARAM_RRE : CLM_NPARAM_EH;
+    if (cfg->model == CLM_MODEL_ENERGY_HYDROLOGY &&
+        (cfg->bc_top == CLM_BC_TOPMODEL || cfg->bc_top == CLM_BC_SURFACE_RUNOFF))
+        return set_err(nullptr, CLM_E_ARG,
+                       "RichardsAtmosDrivenFluxBC runoff is implemented for RichardsModel only");
+    int need = cfg->model == CLM_MODEL_RICHARDS
+                   ? (cfg->bc_top == CLM_BC_TOPMODEL ? CLM_NPARAM_RRE_TOPMODEL : CLM_NPARAM_RRE)
+                   : CLM_NPARAM_EH;
     if (cfg->nparam < need)
         return set_err(nullptr, CLM_E_ARG, "nparam %d < %d required", cfg->nparam, need);
     if (cfg->ncol <= 0 || cfg->ncol > (int64_t)1 << 30 || !cfg->col_ptr || !cfg->z_face ||
@@ -847,6 +855,10 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     ALLOC(c->d_K, c->npcell);
     ALLOC(c->d_psi, c->npcell);
     ALLOC(c->d_tw, c->nslot);
+    if (c->bc_top == CLM_BC_TOPMODEL) {
+        ALLOC(c->d_sink, c->npcell);
+        ALLOC(c->d_rss, c->nslot);
+    }
     ALLOC(c->d_th, c->nslot);
     ALLOC(c->d_bw, c->nslot);
     ALLOC(c->d_bh, c->nslot);
@@ -1006,6 +1018,8 @@ int clm_destroy(clm_ctx* c) {
     dfree(c->d_T);
     dfree(c->d_kap);
     dfree(c->d_tw);
+    dfree(c->d_sink);
+    dfree(c->d_rss);
     dfree(c->d_th);
     dfree(c->d_bw);
     dfree(c->d_bh);
@@ -1141,7 +1155,7 @@ int clm_update_cache(clm_ctx* c, const clm_state* Y, double t) {
     if (!is_eh(c)) {
         hipLaunchKernelGGL(k_rre_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_par, c->d_zf, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw,
-                           c->d_tw, c->d_bw, rre_bc(c), 1);
+                           c->d_tw, c->d_bw, rre_bc(c), 1, c->d_sink, c->d_rss);
     } else {
         hipLaunchKernelGGL(k_eh_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_par, c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_thl, c->d_kap,
@@ -1167,7 +1181,7 @@ int clm_update_implicit_cache(clm_ctx* c, const clm_state* Y, double t) {
         hipLaunchKernelGGL(k_rre_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_par, c->d_zf, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw,
                            c->d_tw, c->d_bw, rre_bc(c),
-                           c->bc_top == CLM_BC_MOISTURE_STATE ? 1 : 0);
+                           c->bc_top == CLM_BC_MOISTURE_STATE ? 1 : 0, c->d_sink, c->d_rss);
     } else {
         hipLaunchKernelGGL(k_eh_imp_cache, grid_slots(c), dim3(256), 0, c->stream, L, c->d_par,
                            c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_T, c->d_psi);
@@ -1218,7 +1232,7 @@ int clm_imp_tendency(clm_ctx* c, const clm_state* Y, double t, clm_state* dY) {
     if (!is_eh(c)) {
         hipLaunchKernelGGL(k_rre_imp_tendency, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_zf, c->d_K, c->d_psi, c->d_tw, c->d_bw, c->d_wd[0],
-                           c->d_wd[3]);
+                           c->d_wd[3], c->d_sink, c->d_rss);
     } else {
         hipLaunchKernelGGL(k_eh_imp_tendency, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_zf, c->d_K, c->d_psi, c->d_T, c->d_kap, c->d_tw, c->d_th,
@@ -1318,6 +1332,10 @@ int clm_get_aux(clm_ctx* c, clm_aux* p) {
     if ((r = download_cells(c, c->d_psi, p->psi))) return r;
     if ((r = download_cols(c, c->d_tw, p->top_water))) return r;
     if ((r = download_cols(c, c->d_bw, p->bottom_water))) return r;
+    if (c->d_sink) {
+        if ((r = download_cells(c, c->d_sink, p->sink))) return r;
+        if ((r = download_cols(c, c->d_rss, p->r_ss))) return r;
+    }
     if (is_eh(c)) {
         if ((r = download_cells(c, c->d_thl, p->theta_l))) return r;
         if ((r = download_cells(c, c->d_T, p->T))) return r;

@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define CLM_ABI_VERSION 1
+#define CLM_ABI_VERSION 2
 
 enum {
     CLM_OK = 0,
@@ -61,17 +61,27 @@ enum { CLM_MODEL_RICHARDS = 0, CLM_MODEL_ENERGY_HYDROLOGY = 1 };
 
 /* Water boundary conditions (boundary_conditions.jl).
  * Top:    CLM_BC_FLUX (WaterFluxBC, :168-180; EH: WaterHeatBC(WaterFluxBC,
- *         HeatFluxBC), :465-477, 709-724) or CLM_BC_MOISTURE_STATE (RRE).
+ *         HeatFluxBC), :465-477, 709-724), CLM_BC_MOISTURE_STATE (RRE), or,
+ *         for RRE, RichardsAtmosDrivenFluxBC(precipitation, runoff) (:79-212)
+ *         with TOPMODELRunoff (CLM_BC_TOPMODEL) or SurfaceRunoff
+ *         (CLM_BC_SURFACE_RUNOFF); with NoRunoff it is CLM_BC_FLUX with the
+ *         precipitation as the flux.
  * Bottom: CLM_BC_FLUX, CLM_BC_FREE_DRAINAGE or CLM_BC_MOISTURE_STATE (RRE).
  * For CLM_BC_MOISTURE_STATE (MoistureStateBC, :227-325) the forcing value is
  * the boundary ϑ_l instead of a flux; with a top MoistureStateBC both
  * boundary fluxes follow the Newton iterate (rre.jl:460-468) and the
- * Jacobian's top row carries set_dfluxBCdY!'s term (:375-411). */
+ * Jacobian's top row carries set_dfluxBCdY!'s term (:375-411).
+ * For the runoff kinds the forcing value is the liquid precipitation rate
+ * P_liq (negative downward); infiltration, and TOPMODEL's subsurface sink
+ * (TOPMODELSubsurfaceRunoff, an implicit source: Runoff/Runoff.jl:174-183,
+ * 321-332), are evaluated at u_n and held through the step. */
 enum {
     CLM_BC_FLUX = 0,           /* prescribed fluxes */
     CLM_BC_FREE_DRAINAGE = 1,  /* FreeDrainage (RRE, :340-353) or
                                   EnergyWaterFreeDrainage (EH, :590-608) */
-    CLM_BC_MOISTURE_STATE = 2  /* MoistureStateBC (RRE) */
+    CLM_BC_MOISTURE_STATE = 2, /* MoistureStateBC (RRE) */
+    CLM_BC_TOPMODEL = 3,       /* top, RRE: TOPMODELRunoff (Runoff.jl:199-271) */
+    CLM_BC_SURFACE_RUNOFF = 4  /* top, RRE: SurfaceRunoff (Runoff.jl:84-148) */
 };
 
 /* Per-column parameter rows.  RRE uses the first CLM_NPARAM_RRE rows
@@ -95,6 +105,10 @@ enum {
     CLM_NPARAM_EH
 };
 #define CLM_NPARAM_RRE 6
+/* RRE under CLM_BC_TOPMODEL: three more rows, TOPMODELRunoff's f_max (per
+ * column), f_over (1/m) and R_sb (m/s) */
+enum { CLM_P_F_MAX = 6, CLM_P_F_OVER = 7, CLM_P_R_SB = 8 };
+#define CLM_NPARAM_RRE_TOPMODEL 9
 
 /* Coarse-path solvers (clm_step / clm_run).  Both solve the same
  * block-tridiagonal Newton systems; they differ in how a column maps onto
@@ -122,7 +136,8 @@ typedef struct {
                                  > 0: per-column stop when max|Δx| < tol over
                                  the cell unknowns (ρe_int scaled by 1/ρc_s) */
     int32_t solver;           /* CLM_SOLVER_*: how clm_step / clm_run solve */
-    int32_t bc_top;           /* CLM_BC_FLUX or CLM_BC_MOISTURE_STATE */
+    int32_t bc_top;           /* CLM_BC_FLUX, CLM_BC_MOISTURE_STATE, CLM_BC_TOPMODEL or
+                                 CLM_BC_SURFACE_RUNOFF */
 } clm_config;
 
 /* Prognostic state Y.soil.  RRE: vartheta_l, intF_water.
@@ -149,6 +164,9 @@ typedef struct {
     double* top_heat;    /* p.soil.top_bc.heat */
     double* bottom_water;
     double* bottom_heat;
+    double* sink;        /* CLM_BC_TOPMODEL: subsurface runoff sink per cell (1/s),
+                            (R_ss / max(h∇, eps)) · is_saturated (Runoff.jl:321-332) */
+    double* r_ss;        /* CLM_BC_TOPMODEL: p.soil.R_ss per column (m/s) */
 } clm_aux;
 
 typedef struct {

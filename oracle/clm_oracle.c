@@ -301,6 +301,7 @@ static inline double dzf(const col_geom* g, int i) { return zc(g, i) - zc(g, i -
 typedef struct {
     double nu, theta_r, alpha, n, Ksat, Ss;
     double nu_om, nu_q, nu_g, k_dry, k_satf, k_satu, rhoc_ds;
+    double f_max, f_over, R_sb; /* TOPMODEL */
 } col_par;
 
 static void load_par(const orc_model* m, int64_t c, col_par* q) {
@@ -323,6 +324,12 @@ static void load_par(const orc_model* m, int64_t c, col_par* q) {
     } else {
         q->nu_om = q->nu_q = q->nu_g = q->k_dry = q->k_satf = q->k_satu = q->rhoc_ds = 0.0;
     }
+    q->f_max = q->f_over = q->R_sb = 0.0;
+    if (m->bc_top == ORC_BC_TOPMODEL) {
+        q->f_max = P[ORC_P_F_MAX * nc + c];
+        q->f_over = P[ORC_P_F_OVER * nc + c];
+        q->R_sb = P[ORC_P_R_SB * nc + c];
+    }
 }
 
 /* ---- per-column views ------------------------------------------------- */
@@ -332,8 +339,8 @@ typedef struct {
 } col_state;
 
 typedef struct {
-    double *K, *psi, *thl, *T, *kap; /* cells */
-    double *tw, *th, *bw, *bh;       /* scalars */
+    double *K, *psi, *thl, *T, *kap, *sink; /* cells */
+    double *tw, *th, *bw, *bh, *rss;        /* scalars */
 } col_aux;
 
 /* ---- RichardsModel ---------------------------------------------------- */
@@ -353,15 +360,52 @@ static void rre_aux(const col_geom* g, const col_par* q, const double* vl, doubl
 static inline double dztop(const col_geom* g) { return g->zf[g->N] - zc(g, g->N - 1); }
 static inline double dzbot(const col_geom* g) { return zc(g, 0) - g->zf[0]; }
 
+/* TOPMODEL runoff for RichardsModel (Runoff/Runoff.jl:234-271, 321-344,
+ * 373-440; Niu et al. 2005 eqs 8, 11, 12) at the state vl:
+ *   is_sat_i = H((ϑ_i - θ_r) - (ν - θ_r)) (ϑ_i - θ_r)/(ν - θ_r),  H(x) = 1 iff x > eps
+ *   h∇ = Σ is_sat_i Δz_i (column_integral_definite!), z∇ = depth - h∇
+ *   f_sat = min(f_max exp(-f_over z∇ / 2), 1),  i_c = -K_sat
+ *   infiltration = (1 - f_sat) max(i_c, P),  R_ss = R_sb exp(-f_over z∇)
+ *   sink_i = R_ss / max(h∇, eps) is_sat_i   (dϑ_i -= sink_i, d∫F -= R_ss)
+ * With no ice the surface and subsurface passes see the same saturation. */
+static void rre_topmodel(const col_geom* g, const col_par* q, const double* vl, double precip,
+                         double* infil, double* r_ss, double* sink) {
+    int N = g->N;
+    double nsat = q->nu - q->theta_r, h = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double twc = vl[i] - q->theta_r;
+        double s = orc_heaviside(twc - nsat) * twc / nsat;
+        if (sink) sink[i] = s;
+        h += s * dzc(g, i);
+    }
+    double zw = (g->zf[N] - g->zf[0]) - h;
+    double fsat = fmin(q->f_max * EXP(-q->f_over / 2.0 * zw), 1.0);
+    double ic = -1.0 * q->Ksat;
+    *infil = (1.0 - fsat) * fmax(ic, precip);
+    *r_ss = q->R_sb * EXP(-q->f_over * zw);
+    if (sink) {
+        double scale = *r_ss / fmax(h, DBL_EPSILON);
+        for (int i = 0; i < N; ++i) sink[i] *= scale;
+    }
+}
+
 /* rre.jl:111-149 boundary fluxes: WaterFluxBC (boundary_conditions.jl:168-180),
  * FreeDrainage (:340-353, -K at level 1) and MoistureStateBC (:227-325):
  * diffusive_flux(K_eff, x_2, x_1, Δz) = -K_eff (x_2 - x_1) / Δz with K_eff the
  * boundary cell's K, ψ_bc = pressure_head(ϑ_bc) with that cell's parameters,
  * and gravity entering as ψ + Δz on the upper point. */
-static void rre_bc(const orc_model* m, const col_geom* g, const col_par* q, const double* K,
-                   const double* psi, double f_top, double f_bot, double* tw, double* bw) {
+static void rre_bc(const orc_model* m, const col_geom* g, const col_par* q, const double* vl,
+                   const double* K, const double* psi, double f_top, double f_bot, double* tw,
+                   double* bw, double* sink, double* r_ss) {
     int N = g->N;
-    if (m->bc_top == ORC_BC_MOISTURE_STATE) {
+    if (m->bc_top == ORC_BC_TOPMODEL) {
+        double rs = 0.0;
+        rre_topmodel(g, q, vl, f_top, tw, &rs, sink);
+        if (r_ss) *r_ss = rs;
+    } else if (m->bc_top == ORC_BC_SURFACE_RUNOFF) {
+        /* Runoff.jl:107-148: (1 - H(ϑ_N - ν)) max(-K_sat, P) */
+        *tw = (1.0 - orc_heaviside(vl[N - 1] - q->nu)) * fmax(-1.0 * q->Ksat, f_top);
+    } else if (m->bc_top == ORC_BC_MOISTURE_STATE) {
         double dz = dztop(g);
         double psib = orc_pressure_head(q->alpha, q->n, q->theta_r, f_top, q->nu, q->Ss);
         *tw = -K[N - 1] * ((psib + dz) - psi[N - 1]) / dz;
@@ -401,7 +445,8 @@ static inline double darcy_face(const col_geom* g, const double* K, const double
 
 /* rre.jl:161-203 compute_imp_tendency */
 static void rre_imp_tendency(const col_geom* g, const double* K, const double* psi,
-                             double tw, double bw, double* dvl, double* diw) {
+                             double tw, double bw, const double* sink, double r_ss, double* dvl,
+                             double* diw) {
     int N = g->N;
     *diw = -(tw - bw);
     double Flo = bw;
@@ -409,6 +454,11 @@ static void rre_imp_tendency(const col_geom* g, const double* K, const double* p
         double Fhi = (i == N - 1) ? tw : darcy_face(g, K, psi, i + 1);
         dvl[i] = -(Fhi - Flo) / dzc(g, i);
         Flo = Fhi;
+    }
+    /* implicit sources (rre.jl:195-200): TOPMODEL subsurface runoff */
+    if (sink) {
+        for (int i = 0; i < N; ++i) dvl[i] -= sink[i];
+        *diw -= r_ss;
     }
 }
 
@@ -633,12 +683,12 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
     load_par(m, c, &q);
     double* w = scratch_get(S, 14 * N + 8);
     double *K = w, *psi = w + N, *U0 = w + 2 * N, *R = w + 3 * N, *dx = w + 4 * N,
-           *W = w + 5 * N, *tmp = w + 8 * N, *D = w + 10 * N, *T = w + 11 * N;
-    (void)T;
-    double tw, bw;
+           *W = w + 5 * N, *tmp = w + 8 * N, *D = w + 10 * N, *sink = w + 11 * N;
+    double tw, bw, rss = 0.0;
+    const int topm = m->bc_top == ORC_BC_TOPMODEL;
     /* stage 1 */
     rre_aux(&g, &q, Y->vl, K, psi);
-    rre_bc(m, &g, &q, K, psi, f_top, f_bot, &tw, &bw);
+    rre_bc(m, &g, &q, Y->vl, K, psi, f_top, f_bot, &tw, &bw, topm ? sink : NULL, &rss);
     const int imp_bc = m->bc_top == ORC_BC_MOISTURE_STATE;
     /* exp tendency is zero (rre.jl:214-243): U0 = u_n */
     memcpy(U0, Y->vl, sizeof(double) * N);
@@ -647,7 +697,7 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
     if (gv) { /* warm start x0 = U0 + guess; cache_imp!(x0) */
         for (int i = 0; i < N; ++i) x[i] += gv[i];
         rre_aux(&g, &q, x, K, psi);
-        if (imp_bc) rre_bc(m, &g, &q, K, psi, f_top, f_bot, &tw, &bw);
+        if (imp_bc) rre_bc(m, &g, &q, x, K, psi, f_top, f_bot, &tw, &bw, NULL, NULL);
     }
     int k;
     for (k = 1; k <= max_iters; ++k) {
@@ -658,7 +708,7 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
         rre_top_bc_jacobian(m, &g, K, D, dt, W);
         /* residual */
         double diw;
-        rre_imp_tendency(&g, K, psi, tw, bw, R, &diw);
+        rre_imp_tendency(&g, K, psi, tw, bw, topm ? sink : NULL, rss, R, &diw);
         double err = 0.0;
         for (int i = 0; i < N; ++i) R[i] = U0[i] + dt * R[i] - x[i];
         double Riw = U0iw + dt * diw - *Y->iw;
@@ -677,7 +727,7 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
          * fluxes are refreshed only under a top MoistureStateBC
          * (rre.jl:460-468), otherwise they stay lagged at u_n */
         rre_aux(&g, &q, x, K, psi);
-        if (imp_bc) rre_bc(m, &g, &q, K, psi, f_top, f_bot, &tw, &bw);
+        if (imp_bc) rre_bc(m, &g, &q, x, K, psi, f_top, f_bot, &tw, &bw, NULL, NULL);
         if (conv) break;
     }
     if (k > max_iters) {
@@ -784,6 +834,8 @@ static void view_aux(const orc_model* m, const orc_aux* p, int64_t c, col_aux* v
     v->psi = p->psi + a;
     v->tw = p->top_water + c;
     v->bw = p->bot_water + c;
+    v->sink = (m->bc_top == ORC_BC_TOPMODEL && p->sink) ? p->sink + a : NULL;
+    v->rss = (m->bc_top == ORC_BC_TOPMODEL && p->r_ss) ? p->r_ss + c : NULL;
     if (m->model == ORC_MODEL_ENERGY_HYDROLOGY) {
         v->thl = p->theta_l + a;
         v->T = p->T + a;
@@ -811,8 +863,8 @@ int orc_update_cache(const orc_model* m, const orc_state* Y, const orc_forcing* 
         load_par(m, c, &q);
         if (m->model == ORC_MODEL_RICHARDS) {
             rre_aux(&g, &q, y.vl, a.K, a.psi);
-            rre_bc(m, &g, &q, a.K, a.psi, fval(f->top_water, c), fval(f->bot_water, c), a.tw,
-                   a.bw);
+            rre_bc(m, &g, &q, y.vl, a.K, a.psi, fval(f->top_water, c), fval(f->bot_water, c),
+                   a.tw, a.bw, a.sink, a.rss);
         } else {
             eh_aux(&g, &q, y.vl, y.ti, y.re, a.thl, a.kap, a.T, a.K, a.psi);
             eh_bc(m->bc_bottom, a.K, a.T, fval(f->top_water, c), fval(f->top_heat, c),
@@ -837,8 +889,8 @@ int orc_update_implicit_cache(const orc_model* m, const orc_state* Y, const orc_
         if (m->model == ORC_MODEL_RICHARDS) {
             rre_aux(&g, &q, y.vl, a.K, a.psi);
             if (m->bc_top == ORC_BC_MOISTURE_STATE && f)
-                rre_bc(m, &g, &q, a.K, a.psi, fval(f->top_water, c), fval(f->bot_water, c),
-                       a.tw, a.bw);
+                rre_bc(m, &g, &q, y.vl, a.K, a.psi, fval(f->top_water, c),
+                       fval(f->bot_water, c), a.tw, a.bw, NULL, NULL);
         } else {
             eh_imp_aux(&g, &q, y.vl, y.ti, y.re, a.T, a.psi);
         }
@@ -881,7 +933,8 @@ int orc_imp_tendency(const orc_model* m, const orc_state* Y, const orc_aux* p,
         int32_t o = m->col_ptr[c];
         col_geom g = {m->col_ptr[c + 1] - o, m->z_face + o + c};
         if (m->model == ORC_MODEL_RICHARDS)
-            rre_imp_tendency(&g, a.K, a.psi, *a.tw, *a.bw, d.vl, d.iw);
+            rre_imp_tendency(&g, a.K, a.psi, *a.tw, *a.bw, a.sink, a.rss ? *a.rss : 0.0, d.vl,
+                             d.iw);
         else
             eh_imp_tendency(&g, a.K, a.psi, a.T, a.kap, *a.tw, *a.th, *a.bw, *a.bh, d.vl, d.ti,
                             d.re, d.iw, d.ie);

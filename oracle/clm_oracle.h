@@ -38,6 +38,9 @@ enum {
     ORC_NPARAM_EH
 };
 #define ORC_NPARAM_RRE 6
+/* RichardsModel with a TOPMODEL top boundary takes three more rows */
+enum { ORC_P_F_MAX = 6, ORC_P_F_OVER = 7, ORC_P_R_SB = 8 };
+#define ORC_NPARAM_RRE_TOPMODEL 9
 
 enum { ORC_MODEL_RICHARDS = 0, ORC_MODEL_ENERGY_HYDROLOGY = 1 };
 /* Water boundary conditions.  Bottom: prescribed flux (WaterFluxBC /
@@ -45,7 +48,15 @@ enum { ORC_MODEL_RICHARDS = 0, ORC_MODEL_ENERGY_HYDROLOGY = 1 };
  * or MoistureStateBC (RRE).  Top: prescribed flux (WaterFluxBC, with
  * HeatFluxBC for EH) or MoistureStateBC (RRE).  For a MoistureStateBC the
  * forcing value is the boundary ϑ_l, not a flux. */
-enum { ORC_BC_FLUX = 0, ORC_BC_FREE_DRAINAGE = 1, ORC_BC_MOISTURE_STATE = 2 };
+enum { ORC_BC_FLUX = 0, ORC_BC_FREE_DRAINAGE = 1, ORC_BC_MOISTURE_STATE = 2,
+       /* top only, RRE: RichardsAtmosDrivenFluxBC(precipitation, TOPMODELRunoff)
+        * (boundary_conditions.jl:79-212, Runoff/Runoff.jl:174-486); the forcing
+        * value is the liquid precipitation rate; parameter rows
+        * ORC_P_F_MAX, ORC_P_F_OVER, ORC_P_R_SB */
+       ORC_BC_TOPMODEL = 3,
+       /* top only, RRE: RichardsAtmosDrivenFluxBC(precipitation, SurfaceRunoff)
+        * (Runoff.jl:84-148); the forcing value is the precipitation rate */
+       ORC_BC_SURFACE_RUNOFF = 4 };
 
 typedef struct {
     int32_t model;
@@ -78,6 +89,8 @@ typedef struct {
     double* top_heat;
     double* bot_water;
     double* bot_heat;
+    double* sink;      /* TOPMODEL subsurface runoff sink per cell (1/s), may be NULL */
+    double* r_ss;      /* TOPMODEL subsurface runoff R_ss per column (m/s), may be NULL */
 } orc_aux;
 
 /* prescribed fluxes (per column), bottom ones used only for ORC_BC_FLUX */

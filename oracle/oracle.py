@@ -31,7 +31,7 @@ class Model(C.Structure):
                 ("reserved_", C.c_int32)]
 
 
-BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE = 0, 1, 2
+BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE, BC_TOPMODEL, BC_SURFACE_RUNOFF = 0, 1, 2, 3, 4
 
 
 class State(C.Structure):
@@ -41,7 +41,8 @@ class State(C.Structure):
 
 class Aux(C.Structure):
     _fields_ = [("K", _dp), ("psi", _dp), ("theta_l", _dp), ("T", _dp), ("kappa", _dp),
-                ("top_water", _dp), ("top_heat", _dp), ("bot_water", _dp), ("bot_heat", _dp)]
+                ("top_water", _dp), ("top_heat", _dp), ("bot_water", _dp), ("bot_heat", _dp),
+                ("sink", _dp), ("r_ss", _dp)]
 
 
 class Forcing(C.Structure):
@@ -167,14 +168,15 @@ class OracleModel:
                                                "intF_water", "intF_energy")))
 
     def new_aux(self) -> dict:
-        a = {k: np.zeros(self.ncell) for k in ("K", "psi", "theta_l", "T", "kappa")}
+        a = {k: np.zeros(self.ncell) for k in ("K", "psi", "theta_l", "T", "kappa", "sink")}
         a.update({k: np.zeros(self.ncol) for k in ("top_water", "top_heat", "bot_water",
-                                                   "bot_heat")})
+                                                   "bot_heat", "r_ss")})
         return a
 
     def _a(self, a: dict) -> Aux:
-        return Aux(*(_p(a[k]) for k in ("K", "psi", "theta_l", "T", "kappa", "top_water",
-                                        "top_heat", "bot_water", "bot_heat")))
+        return Aux(*(_p(a.get(k)) for k in ("K", "psi", "theta_l", "T", "kappa", "top_water",
+                                            "top_heat", "bot_water", "bot_heat", "sink",
+                                            "r_ss")))
 
     @staticmethod
     def _f(tw, th=None, bw=None, bh=None) -> Forcing:

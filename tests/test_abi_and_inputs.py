@@ -36,9 +36,9 @@ def test_library_exports_every_declared_symbol():
 def test_header_enums_match_binding():
     from clmsoil import _lib
     txt = open(HEADER).read()
-    assert "CLM_ABI_VERSION 1" in txt
+    assert "CLM_ABI_VERSION 2" in txt
     lib = _lib.load()
-    assert lib.clm_abi_version() == 1
+    assert lib.clm_abi_version() == 2
     assert "CLM_NPARAM_RRE 6" in txt and _lib.NPARAM_RRE == 6 and _lib.NPARAM_EH == 13
     block = txt.split("CLM_P_NU = 0,")[1].split("};")[0]
     order = re.findall(r"CLM_P_(\w+)", block)
@@ -51,7 +51,7 @@ def test_struct_layouts_match_header():
     from clmsoil import _lib
     assert C.sizeof(_lib.Config) == 4 + 4 + 8 + 8 * 3 + 4 * 3 + 4 + 8 + 4 + 4  # incl. padding
     assert C.sizeof(_lib.State) == 5 * 8
-    assert C.sizeof(_lib.Aux) == 9 * 8
+    assert C.sizeof(_lib.Aux) == 11 * 8
     assert C.sizeof(_lib.StepStats) == 4 + 4 + 8 + 8 * 3 + 8
 
 
@@ -65,6 +65,10 @@ def test_struct_layouts_match_header():
     (lambda c: setattr(c, "bc_top", 1), -1),          # free drainage is a bottom condition
     (lambda c: setattr(c, "bc_top", 7), -1),
     (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 2)), -1),
+    (lambda c: setattr(c, "bc_top", 3), -1),          # TOPMODEL needs 9 parameter rows
+    (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 3)), -1),
+    (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 4)), -1),
+    (lambda c: setattr(c, "bc_top", 5), -1),
 ])
 def test_create_rejects_bad_config_without_device(mutate, code):
     """Validation runs before any HIP call, so this is safe without a GPU."""
