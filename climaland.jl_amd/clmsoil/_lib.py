@@ -32,7 +32,7 @@ EXPORTED = (
     "clm_totals", "clm_last_kernel_ms", "clm_get_shapes", "clm_set_shape", "clm_autotune",
     "clm_ws_init", "clm_ws_dims", "clm_ws_get_params", "clm_ws_set_params", "clm_ws_enable",
     "clm_ws_collect", "clm_ws_train", "clm_ws_features", "clm_ws_forward", "clm_ws_gradient",
-    "clm_ws_time", "clm_save_checkpoint", "clm_read_checkpoint",
+    "clm_ws_time", "clm_ws_guess", "clm_save_checkpoint", "clm_read_checkpoint",
 )
 
 _dp = C.POINTER(C.c_double)
@@ -124,6 +124,7 @@ def load(path: str | None = None) -> C.CDLL:
         "clm_ws_forward": ([vp, _fp, C.c_int64, _fp], C.c_int),
         "clm_ws_gradient": ([vp, _fp, _fp, _fp, C.c_int64, _fp, _fp], C.c_int),
         "clm_ws_time": ([vp, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
+        "clm_ws_guess": ([vp, C.c_double, C.c_int32, _dp, _dp], C.c_int),
         "clm_save_checkpoint": ([vp, C.c_char_p, C.c_double, C.c_char_p, C.c_int32], C.c_int),
         "clm_read_checkpoint": ([vp, C.c_char_p, C.POINTER(C.c_double), _ip], C.c_int),
     }

@@ -446,6 +446,14 @@ class NewtonWarmStart:
                                                          L.fptr(Y)))
         return Y
 
+    def guess(self, dt: float, fused: bool = True):
+        """Per-cell guesses (ϑ_l, ρe_int or None) for the device state."""
+        m = self.model
+        gv = np.zeros(m.domain.ncell)
+        ge = np.zeros(m.domain.ncell) if m.is_eh else None
+        m._check(m._lib.clm_ws_guess(m._ctx, dt, int(fused), L.dptr(gv), L.dptr(ge)))
+        return gv, ge
+
     def inference_ms(self, dt: float, reps: int = 5) -> float:
         """Device time of one inference (features, network, guess), in ms."""
         ms = C.c_double(0.0)

@@ -94,18 +94,17 @@ __device__ __forceinline__ double tail_feature(int i, bool eh, double al, double
 }
 
 // ---- features of the current state: 32 lanes per column -----------------------
-__global__ __launch_bounds__(256) void k_features(
-    bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
-    const double* __restrict__ xv, const double* __restrict__ xi, const double* __restrict__ xe,
-    const double* __restrict__ ftw, const double* __restrict__ fth, double dt, int Fp,
-    bf16* __restrict__ X) {
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t c = gt / kGroup;
-    const int j = (int)(gt % kGroup);
-    if (c >= ncol) return;
+// lane j of column c's group writes entries j, 32 + j, ... of the column's row
+__device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict__ col_ptr,
+                                             int64_t ncol, const double* __restrict__ par,
+                                             const double* __restrict__ xv,
+                                             const double* __restrict__ xi,
+                                             const double* __restrict__ xe,
+                                             const double* __restrict__ ftw,
+                                             const double* __restrict__ fth, double dt, int Fp,
+                                             int64_t c, int j, bf16* __restrict__ row) {
     const int64_t b = col_ptr[c];
     const int N = (int)(col_ptr[c + 1] - b);
-    bf16* row = X + c * (int64_t)Fp;
     const double nu = par[P_NU * ncol + c], thr = par[P_THETA_R * ncol + c];
     row[j] = feat((at_node(j, N, [&](int i) { return xv[b + i]; }) - thr) / (nu - thr));
     int base = kNodes;
@@ -127,6 +126,18 @@ __global__ __launch_bounds__(256) void k_features(
                                                thr, ftw[c], eh ? fth[c] : 0.0, dt, N))
                            : (bf16)0.0f;
     }
+}
+
+__global__ __launch_bounds__(256) void k_features(
+    bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
+    const double* __restrict__ xv, const double* __restrict__ xi, const double* __restrict__ xe,
+    const double* __restrict__ ftw, const double* __restrict__ fth, double dt, int Fp,
+    bf16* __restrict__ X) {
+    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = gt / kGroup;
+    const int j = (int)(gt % kGroup);
+    if (c >= ncol) return;
+    feature_lane(eh, col_ptr, ncol, par, xv, xi, xe, ftw, fth, dt, Fp, c, j, X + c * (int64_t)Fp);
 }
 
 // ---- training targets from (u_n, U0_ϑ, x_{n+1}): 32 lanes per column ----------
@@ -170,17 +181,14 @@ __global__ __launch_bounds__(256) void k_targets(
 }
 
 // ---- network output -> per-cell guess (FP64, CSR): 32 lanes per column ---------
-__global__ __launch_bounds__(256) void k_guess(
-    bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
-    const double* __restrict__ xv, const double* __restrict__ xi, const float* __restrict__ Y,
-    int Nout, double* __restrict__ gv, double* __restrict__ ge) {
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t c = gt / kGroup;
-    const int j = (int)(gt % kGroup);
-    if (c >= ncol) return;
+__device__ __forceinline__ void guess_lane(bool eh, const int32_t* __restrict__ col_ptr,
+                                           int64_t ncol, const double* __restrict__ par,
+                                           const double* __restrict__ xv,
+                                           const double* __restrict__ xi,
+                                           const float* __restrict__ o, int64_t c, int j,
+                                           double* __restrict__ gv, double* __restrict__ ge) {
     const int64_t b = col_ptr[c];
     const int N = (int)(col_ptr[c + 1] - b);
-    const float* o = Y + c * (int64_t)Nout;
     const double nu = eh ? par[P_NU * ncol + c] : 0.0;
     const double rcd = eh ? par[P_RHOC_DS * ncol + c] : 0.0;
     for (int i = j; i < N; i += kGroup) {
@@ -190,6 +198,17 @@ __global__ __launch_bounds__(256) void k_guess(
             ge[b + i] = at_cell(i, N, o + kNodes) * kScaleT * heat_capacity(fmin(nu - t, v), t, rcd);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_guess(
+    bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
+    const double* __restrict__ xv, const double* __restrict__ xi, const float* __restrict__ Y,
+    int Nout, double* __restrict__ gv, double* __restrict__ ge) {
+    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t c = gt / kGroup;
+    const int j = (int)(gt % kGroup);
+    if (c >= ncol) return;
+    guess_lane(eh, col_ptr, ncol, par, xv, xi, Y + c * (int64_t)Nout, c, j, gv, ge);
 }
 
 // ---- GEMM: C[M x N] = A[M x K] . B[N x K]^T (both K-contiguous), BF16 in, FP32 acc
@@ -468,6 +487,170 @@ __global__ __launch_bounds__(512) void k_gemm_wide(
     }
 }
 
+// ---- fused inference: features -> 4 layers -> per-cell guess in one kernel -----
+// A 512-thread block takes 128 columns.  Their feature rows are built in LDS
+// (the same lanes and arithmetic as k_features), pass through the four layers
+// with the activations kept in LDS (one [128][H + 8] BF16 buffer, overwritten
+// by each hidden layer's output once every wave has finished reading it), and
+// the output rows become per-cell guesses as in k_guess.  HBM sees the
+// state, the parameters and the guesses only.  Weights stream from L2 in
+// chunk-major order ([K/32][N][32], written by k_cast_weights / k_adam) through
+// a double-buffered [2][N][32 + 8] LDS buffer, one barrier per 32-wide K chunk.
+// Each layer keeps the accumulation order of k_gemm_wide / k_gemm_nt (K chunks
+// of 32 ascending, two 16-deep MFMA steps per chunk, the same transposed
+// operand order), so the guesses equal the unfused path's bit for bit.
+constexpr int kIBM = 128;
+
+template <bool EH>
+struct NetDims;
+template <>
+struct NetDims<true> {
+    static constexpr int Fp = 128, H = 384, No = 64;
+};
+template <>
+struct NetDims<false> {
+    static constexpr int Fp = 64, H = 256, No = 32;
+};
+
+// one layer over the block's 128 rows: Act[:, 0:K] . W^T + bias; hidden layers
+// write GELU(.) as BF16 into Act[:, 0:N], the output layer writes FP32 to Y
+// ([128][N] in LDS, aliasing Wb once the K loop is done)
+template <int N, int K, bool HIDDEN>
+__device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
+                                            bf16* __restrict__ Wb,
+                                            const bf16* __restrict__ Wpk,
+                                            const float* __restrict__ bias, float* Y) {
+    constexpr int WN = N / 32 < 4 ? N / 32 : 4;          // waves across columns
+    constexpr int WM = 8 / WN < 4 ? 8 / WN : 4;          // waves across rows
+    constexpr int TM = 4 / WM, TN = N / (32 * WN);       // 32x32 tiles per wave
+    constexpr int NU = N * kBK / 8;                       // uint4 per weight chunk
+    constexpr int NB = (NU + 511) / 512;
+    constexpr int WS = kLdsRow;                           // weight chunk row stride
+    static_assert(K % kBK == 0 && N % 32 == 0 && TN >= 1, "layer shape");
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w / WN, wn = w % WN;
+    const bool active = w < WM * WN;
+    floatx16 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    uint4 rb[NB];
+    auto gload = [&](int ch) {
+        const uint4* src = (const uint4*)(Wpk + (int64_t)ch * N * kBK);
+#pragma unroll
+        for (int h = 0; h < NB; ++h) {
+            int idx = t + 512 * h;
+            if (idx < NU) rb[h] = src[idx];
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < NB; ++h) {
+            int idx = t + 512 * h;
+            if (idx < NU) *(uint4*)(Wb + buf * N * WS + (idx >> 2) * WS + (idx & 3) * 8) = rb[h];
+        }
+    };
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int ch = 0; ch < K / kBK; ++ch) {
+        const bool more = ch + 1 < K / kBK;
+        if (more) gload(ch + 1);
+        if (active) {
+#pragma unroll
+            for (int ks = 0; ks < kBK; ks += 16) {
+                bf16x8 a[TM], b[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+                    a[i] = *(const bf16x8*)(Act + (wm * TM * 32 + i * 32 + (lane & 31)) * lds_row +
+                                            ch * kBK + ks + (lane >> 5) * 8);
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    b[j] = *(const bf16x8*)(Wb + buf * N * WS +
+                                            (wn * TN * 32 + j * 32 + (lane & 31)) * WS + ks +
+                                            (lane >> 5) * 8);
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j],
+                                                                            0, 0, 0);
+            }
+        }
+        if (more) lstore(buf ^ 1);  // the other buffer was last read before the previous barrier
+        __syncthreads();
+        buf ^= 1;
+    }
+    // every wave is past its last read of Act and Wb
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+            const int m = wm * TM * 32 + i * 32 + (lane & 31);
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int n = wn * TN * 32 + j * 32 + 8 * q + 4 * (lane >> 5);
+                    float4 bb = *(const float4*)(bias + n);
+                    float4 v = {acc[i][j][4 * q] + bb.x, acc[i][j][4 * q + 1] + bb.y,
+                                acc[i][j][4 * q + 2] + bb.z, acc[i][j][4 * q + 3] + bb.w};
+                    if constexpr (HIDDEN) {
+                        bf16x4 hv = {(bf16)gelu(v.x), (bf16)gelu(v.y), (bf16)gelu(v.z),
+                                     (bf16)gelu(v.w)};
+                        *(bf16x4*)(Act + m * lds_row + n) = hv;
+                    } else {
+                        *(float4*)(Y + m * N + n) = v;
+                    }
+                }
+        }
+    }
+    __syncthreads();
+}
+
+// Wpk[l]: chunk-major BF16 weights of layer l; bias[l]: its FP32 bias
+template <bool EH>
+__global__ __launch_bounds__(512, 1) void k_ws_infer(
+    const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
+    const double* __restrict__ xv, const double* __restrict__ xi, const double* __restrict__ xe,
+    const double* __restrict__ ftw, const double* __restrict__ fth, double dt,
+    const bf16* __restrict__ W0, const bf16* __restrict__ W1, const bf16* __restrict__ W2,
+    const bf16* __restrict__ W3, const float* __restrict__ b0, const float* __restrict__ b1,
+    const float* __restrict__ b2, const float* __restrict__ b3, double* __restrict__ gv,
+    double* __restrict__ ge) {
+    using D = NetDims<EH>;
+    constexpr int LR = D::H + 8;  // LDS row stride of the activations
+    __shared__ __attribute__((aligned(16))) bf16 Act[kIBM * LR];
+    __shared__ __attribute__((aligned(16))) bf16 Wb[2 * D::H * kLdsRow];
+    static_assert(kIBM * D::No * sizeof(float) <= sizeof(Wb), "output rows alias Wb");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t c0 = (int64_t)blockIdx.x * kIBM;
+    // features: each wave takes two columns per pass, lane j -> node j
+    for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
+        const int64_t c = c0 + lc;
+        const int j = lane & 31;
+        bf16* row = Act + lc * LR;
+        if (c < ncol) {
+            feature_lane(EH, col_ptr, ncol, par, xv, xi, xe, ftw, fth, dt, D::Fp, c, j, row);
+        } else {
+            for (int f = j; f < D::Fp; f += kGroup) row[f] = (bf16)0.0f;
+        }
+    }
+    __syncthreads();
+    float* Y = (float*)Wb;
+    infer_layer<D::H, D::Fp, true>(Act, LR, Wb, W0, b0, nullptr);
+    infer_layer<D::H, D::H, true>(Act, LR, Wb, W1, b1, nullptr);
+    infer_layer<D::H, D::H, true>(Act, LR, Wb, W2, b2, nullptr);
+    infer_layer<D::No, D::H, false>(Act, LR, Wb, W3, b3, Y);
+    for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
+        const int64_t c = c0 + lc;
+        if (c < ncol) guess_lane(EH, col_ptr, ncol, par, xv, xi, Y + lc * D::No, c, lane & 31, gv, ge);
+    }
+}
+
 // ---- small training kernels ---------------------------------------------------
 // out[c][r] = in[r][c] (bf16), 32 x 32 tiles through LDS
 __global__ __launch_bounds__(256) void k_transpose(int R, int Cn, const bf16* __restrict__ in,
@@ -554,7 +737,7 @@ __global__ __launch_bounds__(256) void k_adam(int N, int K, float* __restrict__ 
                                               const float* __restrict__ gb, float* __restrict__ m1,
                                               float* __restrict__ m2, float lr, float b1t,
                                               float b2t, bf16* __restrict__ Wt_b,
-                                              bf16* __restrict__ W_b) {
+                                              bf16* __restrict__ W_b, bf16* __restrict__ W_p) {
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     int64_t nw = (int64_t)N * K;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -571,6 +754,7 @@ __global__ __launch_bounds__(256) void k_adam(int N, int K, float* __restrict__ 
         int64_t n = i / K, k = i % K;
         Wt_b[i] = (bf16)p;
         W_b[k * N + n] = (bf16)p;
+        W_p[(k / kBK) * N * kBK + n * kBK + k % kBK] = (bf16)p;
     } else {
         bias[i - nw] -= step;
     }
@@ -579,12 +763,14 @@ __global__ __launch_bounds__(256) void k_adam(int N, int K, float* __restrict__ 
 // BF16 copies of FP32 master weights (after init or a host upload)
 __global__ __launch_bounds__(256) void k_cast_weights(int N, int K, const float* __restrict__ W,
                                                       bf16* __restrict__ Wt_b,
-                                                      bf16* __restrict__ W_b) {
+                                                      bf16* __restrict__ W_b,
+                                                      bf16* __restrict__ W_p) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)N * K) return;
     int64_t n = i / K, k = i % K;
     Wt_b[i] = (bf16)W[i];
     W_b[k * N + n] = (bf16)W[i];
+    W_p[(k / kBK) * N * kBK + n * kBK + k % kBK] = (bf16)W[i];
 }
 
 __global__ __launch_bounds__(256) void k_f32_to_bf16(int64_t n, const float* __restrict__ in,

@@ -479,6 +479,7 @@ struct WsState {
     int64_t P = 0, PW = 0;
     float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     bf16 *wt = nullptr, *wn = nullptr;  // BF16 Wt [N x K] and W = Wt^T [K x N]
+    bf16* wpk = nullptr;                // BF16 Wt chunk-major [K/32][N][32] (fused inference)
     int64_t adam_t = 0;
     bool enabled = false;
     // inference (one row per column)
@@ -529,6 +530,7 @@ void ws_free(WsState* w) {
     dfree(w->m2);
     dfree(w->wt);
     dfree(w->wn);
+    dfree(w->wpk);
     dfree(w->X);
     dfree(w->Ha);
     dfree(w->Hb);
@@ -636,7 +638,7 @@ void ws_adam(clm_ctx* c, WsState* w, float lr) {
         hipLaunchKernelGGL(ws::k_adam, grid1(n), dim3(256), 0, c->stream, N, K, P,
                            P + (int64_t)N * K, w->g + w->off[l], w->g + w->off[l] + (int64_t)N * K,
                            w->m1 + w->off[l], w->m2 + w->off[l], lr, b1t, b2t,
-                           w->wt + w->offw[l], w->wn + w->offw[l]);
+                           w->wt + w->offw[l], w->wn + w->offw[l], w->wpk + w->offw[l]);
     }
 }
 
@@ -644,7 +646,8 @@ void ws_cast(clm_ctx* c, WsState* w) {
     for (int l = 0; l < 4; ++l) {
         int64_t n = (int64_t)w->ln[l] * w->lk[l];
         hipLaunchKernelGGL(ws::k_cast_weights, grid1(n), dim3(256), 0, c->stream, w->ln[l],
-                           w->lk[l], w->p + w->off[l], w->wt + w->offw[l], w->wn + w->offw[l]);
+                           w->lk[l], w->p + w->off[l], w->wt + w->offw[l], w->wn + w->offw[l],
+                           w->wpk + w->offw[l]);
     }
 }
 
@@ -675,15 +678,39 @@ int ws_features_launch(clm_ctx* c, WsState* w, const double* ftw, const double* 
     return CLM_OK;
 }
 
-void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
-                     StepExtra* x) {
-    WsState* w = c->ws;
-    if (!w || !w->enabled) return;
+// per-cell guesses of the current state into w->gv / w->ge: one fused kernel,
+// or features, the layer-by-layer GEMMs and k_guess (the same arithmetic)
+void ws_guess(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
+              bool fused) {
+    if (fused) {
+        const bf16* W = w->wpk;
+        const float* P = w->p;
+        auto bias = [&](int l) { return P + w->off[l] + (int64_t)w->ln[l] * w->lk[l]; };
+        dim3 g((unsigned)((c->ncol + ws::kIBM - 1) / ws::kIBM)), b(512);
+        if (is_eh(c))
+            hipLaunchKernelGGL(ws::k_ws_infer<true>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
+                               c->d_cpar, c->d_cv, c->d_ci, c->d_ce, ftw, fth ? fth : ftw, dt,
+                               W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
+                               bias(0), bias(1), bias(2), bias(3), w->gv, w->ge);
+        else
+            hipLaunchKernelGGL(ws::k_ws_infer<false>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
+                               c->d_cpar, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
+                               W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
+                               bias(0), bias(1), bias(2), bias(3), w->gv, w->ge);
+        return;
+    }
     ws_features_launch(c, w, ftw, fth, dt, w->X);
     ws_forward(c, w, w->X, (int)c->ncol, false, w->Y);
     hipLaunchKernelGGL(ws::k_guess, grid1(c->ncol * ws::kGroup), dim3(256), 0, c->stream, is_eh(c),
                        c->d_col_ptr, c->ncol, c->d_cpar, c->d_cv, c->d_ci, w->Y, w->d.Nout,
                        w->gv, w->ge);
+}
+
+void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
+                     StepExtra* x) {
+    WsState* w = c->ws;
+    if (!w || !w->enabled) return;
+    ws_guess(c, w, ftw, fth, dt, true);
     x->gv = w->gv;
     x->ge = is_eh(c) ? w->ge : nullptr;
 }
@@ -1658,6 +1685,7 @@ int clm_ws_init(clm_ctx* c, uint64_t seed) {
     if ((r = dalloc(c, &w->p, w->P)) || (r = dalloc(c, &w->g, w->P)) ||
         (r = dalloc(c, &w->m1, w->P)) || (r = dalloc(c, &w->m2, w->P)) ||
         (r = dalloc(c, &w->wt, w->PW)) || (r = dalloc(c, &w->wn, w->PW)) ||
+        (r = dalloc(c, &w->wpk, w->PW)) ||
         (r = dalloc(c, &w->X, c->ncol * w->d.Fp)) || (r = dalloc(c, &w->Ha, c->ncol * H)) ||
         (r = dalloc(c, &w->Hb, c->ncol * H)) || (r = dalloc(c, &w->Y, c->ncol * w->d.Nout)) ||
         (r = dalloc(c, &w->gv, c->ncell)) || (r = dalloc(c, &w->ge, c->ncell)) ||
@@ -1747,6 +1775,20 @@ int clm_ws_features(clm_ctx* c, double dt, float* out) {
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (e != hipSuccess) return set_err(c, CLM_E_HIP, "features: %s", hipGetErrorString(e));
+    return CLM_OK;
+}
+
+int clm_ws_guess(clm_ctx* c, double dt, int32_t fused, double* gv, double* ge) {
+    if (!c || !gv) return CLM_E_ARG;
+    if (!c->ws) return set_err(c, CLM_E_STATE, "clm_ws_init must run first");
+    WsState* w = c->ws;
+    ws_guess(c, w, c->d_cftw, is_eh(c) ? c->d_cfth : nullptr, dt, fused != 0);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(gv, w->gv, 8 * (size_t)c->ncell, hipMemcpyDeviceToHost, c->stream));
+    if (ge && is_eh(c))
+        HIPCHK(c, hipMemcpyAsync(ge, w->ge, 8 * (size_t)c->ncell, hipMemcpyDeviceToHost,
+                                 c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return CLM_OK;
 }
 

@@ -296,6 +296,12 @@ int clm_ws_features(clm_ctx* ctx, double dt, float* X);
 int clm_ws_forward(clm_ctx* ctx, const float* X, int64_t M, float* Y);
 int clm_ws_gradient(clm_ctx* ctx, const float* X, const float* T, const float* mask, int64_t M,
                     float* loss, float* grad);
+/* The warm start's guess x0 - U0 for the device state and the clm_set_forcing
+ * fluxes, per cell in CSR order: ϑ_l and (EH) ρe_int.  fused = 1 runs the one-
+ * kernel inference that clm_step / clm_run use; fused = 0 the features, layer-
+ * by-layer GEMMs and guess kernels (the same arithmetic, for checking). */
+int clm_ws_guess(clm_ctx* ctx, double dt, int32_t fused, double* guess_vartheta_l,
+                 double* guess_rho_e_int);
 /* Device time of one warm-start inference (features, network, guess) on the
  * current state and clm_set_forcing fluxes, averaged over reps (HIP events). */
 int clm_ws_time(clm_ctx* ctx, double dt, int32_t reps, double* ms_per_call);

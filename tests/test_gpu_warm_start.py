@@ -98,6 +98,51 @@ def test_network_forward_and_gradient_match_cpu(gpu_lib, eh, M):
     m.close()
 
 
+@pytest.mark.parametrize("name", ["rre_131072_uneven", "eh_262144_uneven"])
+def test_fused_inference_equals_layer_path(gpu_lib, name):
+    """The one-kernel inference (features, four layers and guesses with the
+    activations in LDS) against the layer-by-layer kernels: the same
+    accumulation order, so the network outputs and the ϑ_l guesses are equal
+    bit for bit (the ρe_int guesses to an FP64 ulp); and against the CPU
+    restatement (features -> BF16 network -> node-to-cell resampling) to the
+    BF16 bar.  300 columns: the last 128-column block is partial."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart
+    from oracle import mlp_oracle as mo
+    from oracle.oracle import constants
+    b = inputs.make_batch(name, ncol=300)
+    eh = b.top_heat is not None
+    m = _model(b)
+    _set_forcing(m, b)
+    sim = LandSimulation(m, b.copy_state(), b.workload.dt)
+    sim._push()
+    ws = NewtonWarmStart(m, seed=2)
+    rng = np.random.default_rng(4)
+    p = (ws.params() + 0.05 * rng.standard_normal(ws.n_params)).astype(np.float32)
+    ws.set_params(p)
+    dt = b.workload.dt
+    gv1, ge1 = ws.guess(dt, fused=True)
+    gv0, ge0 = ws.guess(dt, fused=False)
+    assert np.isfinite(gv1).all() and np.abs(gv1).max() > 0
+    assert (gv1 == gv0).all()
+    if eh:
+        # the node-to-cell interpolation and ρc_s(u) are FP64 expressions that
+        # the two kernels may contract into FMAs differently: rounding level
+        # against the guesses' scale (interpolation can cancel towards zero)
+        e = np.abs(ge1 - ge0).max() / np.abs(ge0).max()
+        print(f"{name}: energy guess, fused vs layer path max err {e:.2e} of max |guess|",
+              flush=True)
+        assert e <= 1e-14
+    X = mo.features(b, b.copy_state(), b.top_water[0], b.top_heat[0] if eh else None, dt,
+                    constants())
+    Y, _ = mo.forward(p, eh, mo.to_bf16(X.astype(np.float32)))
+    ref = np.concatenate([mo.at_cells(Y[c, :32], n) * 1e-2 for c, n in enumerate(b.nlayers)])
+    e = np.linalg.norm(gv1 - ref) / np.linalg.norm(ref)
+    print(f"{name}: fused guess vs CPU restatement rel L2 {e:.2e}", flush=True)
+    assert e <= 1e-2
+    m.close()
+
+
 def _one_step(b, sim, m):
     w = b.workload
     th = None if b.top_heat is None else b.top_heat[:1]
