@@ -512,19 +512,48 @@ struct NetDims<false> {
     static constexpr int Fp = 64, H = 256, No = 32;
 };
 
+// weight chunk ch ([N][32] of the chunk-major layout) -> registers -> LDS
+template <int N>
+struct Chunk {
+    static constexpr int NU = N * kBK / 8;  // uint4 per chunk
+    static constexpr int NB = (NU + 511) / 512;
+    uint4 r[NB];
+    // threads past the chunk's end repeat its last uint4 (same value, same
+    // LDS slot): every access is unconditional, so r stays in registers and
+    // the loads stay in flight until the store
+    static __device__ __forceinline__ int slot(int h) {
+        int idx = (int)threadIdx.x + 512 * h;
+        return NU % 512 == 0 ? idx : (idx < NU ? idx : NU - 1);
+    }
+    __device__ __forceinline__ void load(const bf16* __restrict__ Wpk, int ch) {
+        const uint4* src = (const uint4*)(Wpk + (int64_t)ch * N * kBK);
+#pragma unroll
+        for (int h = 0; h < NB; ++h) r[h] = src[slot(h)];
+    }
+    __device__ __forceinline__ void store(bf16* __restrict__ Wb, int buf) const {
+#pragma unroll
+        for (int h = 0; h < NB; ++h) {
+            int idx = slot(h);
+            *(uint4*)(Wb + buf * N * kLdsRow + (idx >> 2) * kLdsRow + (idx & 3) * 8) = r[h];
+        }
+    }
+};
+
 // one layer over the block's 128 rows: Act[:, 0:K] . W^T + bias; hidden layers
 // write GELU(.) as BF16 into Act[:, 0:N], the output layer writes FP32 to Y
-// ([128][N] in LDS, aliasing Wb once the K loop is done)
-template <int N, int K, bool HIDDEN>
+// ([128][N] in LDS, aliasing Wb once the K loop is done).  On entry chunk 0
+// of W is in Wb buffer 0 (behind a barrier); on exit, if NN > 0, chunk 0 of
+// the next layer's weights Wnext ([NN][K'] chunk-major) is, its load having
+// overlapped this layer's epilogue.
+template <int N, int K, bool HIDDEN, int NN>
 __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                             bf16* __restrict__ Wb,
                                             const bf16* __restrict__ Wpk,
-                                            const float* __restrict__ bias, float* Y) {
+                                            const float* __restrict__ bias, float* Y,
+                                            const bf16* __restrict__ Wnext) {
     constexpr int WN = N / 32 < 4 ? N / 32 : 4;          // waves across columns
     constexpr int WM = 8 / WN < 4 ? 8 / WN : 4;          // waves across rows
     constexpr int TM = 4 / WM, TN = N / (32 * WN);       // 32x32 tiles per wave
-    constexpr int NU = N * kBK / 8;                       // uint4 per weight chunk
-    constexpr int NB = (NU + 511) / 512;
     constexpr int WS = kLdsRow;                           // weight chunk row stride
     static_assert(K % kBK == 0 && N % 32 == 0 && TN >= 1, "layer shape");
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -537,29 +566,11 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    uint4 rb[NB];
-    auto gload = [&](int ch) {
-        const uint4* src = (const uint4*)(Wpk + (int64_t)ch * N * kBK);
-#pragma unroll
-        for (int h = 0; h < NB; ++h) {
-            int idx = t + 512 * h;
-            if (idx < NU) rb[h] = src[idx];
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int h = 0; h < NB; ++h) {
-            int idx = t + 512 * h;
-            if (idx < NU) *(uint4*)(Wb + buf * N * WS + (idx >> 2) * WS + (idx & 3) * 8) = rb[h];
-        }
-    };
-    gload(0);
-    lstore(0);
-    __syncthreads();
+    Chunk<N> nxt;
     int buf = 0;
     for (int ch = 0; ch < K / kBK; ++ch) {
         const bool more = ch + 1 < K / kBK;
-        if (more) gload(ch + 1);
+        if (more) nxt.load(Wpk, ch + 1);
         if (active) {
 #pragma unroll
             for (int ks = 0; ks < kBK; ks += 16) {
@@ -581,11 +592,17 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                                                             0, 0, 0);
             }
         }
-        if (more) lstore(buf ^ 1);  // the other buffer was last read before the previous barrier
+        if (more) nxt.store(Wb, buf ^ 1);  // the other buffer was last read before the previous barrier
         __syncthreads();
         buf ^= 1;
     }
     // every wave is past its last read of Act and Wb
+    if constexpr (NN > 0) {
+        Chunk<NN> first;
+        first.load(Wnext, 0);  // in flight during the epilogue
+        if constexpr (HIDDEN) first.store(Wb, 0);
+        else static_assert(HIDDEN, "only hidden layers are followed by a layer");
+    }
     if (active) {
 #pragma unroll
         for (int i = 0; i < TM; ++i) {
@@ -628,23 +645,28 @@ __global__ __launch_bounds__(512, 1) void k_ws_infer(
     static_assert(kIBM * D::No * sizeof(float) <= sizeof(Wb), "output rows alias Wb");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * kIBM;
-    // features: each wave takes two columns per pass, lane j -> node j
-    for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
-        const int64_t c = c0 + lc;
-        const int j = lane & 31;
-        bf16* row = Act + lc * LR;
-        if (c < ncol) {
-            feature_lane(EH, col_ptr, ncol, par, xv, xi, xe, ftw, fth, dt, D::Fp, c, j, row);
-        } else {
-            for (int f = j; f < D::Fp; f += kGroup) row[f] = (bf16)0.0f;
+    // features: each wave takes two columns per pass, lane j -> node j.  Rows
+    // past the last column repeat its features (their outputs are never
+    // stored), so every load is unconditional and the unrolled passes can
+    // have their loads in flight together.
+    {
+        Chunk<D::H> first;
+        first.load(W0, 0);  // layer 1's first weight chunk, in flight during the features
+#pragma unroll 4
+        for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
+            const int64_t c = c0 + lc < ncol ? c0 + lc : ncol - 1;
+            feature_lane(EH, col_ptr, ncol, par, xv, xi, xe, ftw, fth, dt, D::Fp, c, lane & 31,
+                         Act + lc * LR);
         }
+        first.store(Wb, 0);
     }
     __syncthreads();
     float* Y = (float*)Wb;
-    infer_layer<D::H, D::Fp, true>(Act, LR, Wb, W0, b0, nullptr);
-    infer_layer<D::H, D::H, true>(Act, LR, Wb, W1, b1, nullptr);
-    infer_layer<D::H, D::H, true>(Act, LR, Wb, W2, b2, nullptr);
-    infer_layer<D::No, D::H, false>(Act, LR, Wb, W3, b3, Y);
+    infer_layer<D::H, D::Fp, true, D::H>(Act, LR, Wb, W0, b0, nullptr, W1);
+    infer_layer<D::H, D::H, true, D::H>(Act, LR, Wb, W1, b1, nullptr, W2);
+    infer_layer<D::H, D::H, true, D::No>(Act, LR, Wb, W2, b2, nullptr, W3);
+    infer_layer<D::No, D::H, false, 0>(Act, LR, Wb, W3, b3, Y, nullptr);
+#pragma unroll 4
     for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
         const int64_t c = c0 + lc;
         if (c < ncol) guess_lane(EH, col_ptr, ncol, par, xv, xi, Y + lc * D::No, c, lane & 31, gv, ge);
