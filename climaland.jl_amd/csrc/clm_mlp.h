@@ -94,9 +94,28 @@ __device__ __forceinline__ double tail_feature(int i, bool eh, double al, double
 }
 
 // ---- features of the current state: 32 lanes per column -----------------------
+// Tail features that depend on the column alone (log10 α, n - 2, K_sat, ν,
+// θ_r, N / 40) are rounded once per context into tails[c][0..5]
+constexpr int kTails = 8;
+__global__ __launch_bounds__(256) void k_static_tails(bool eh, const int32_t* __restrict__ col_ptr,
+                                                      int64_t ncol, const double* __restrict__ par,
+                                                      bf16* __restrict__ tails) {
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    const int N = (int)(col_ptr[c + 1] - col_ptr[c]);
+    bf16* o = tails + c * kTails;
+    for (int i = 0; i < 5; ++i)
+        o[i] = feat(tail_feature(i, eh, par[P_VG_ALPHA * ncol + c], par[P_VG_N * ncol + c],
+                                 par[P_KSAT * ncol + c], par[P_NU * ncol + c],
+                                 par[P_THETA_R * ncol + c], 0.0, 0.0, 0.0, N));
+    o[5] = feat((double)N / 40.0);
+    o[6] = o[7] = (bf16)0.0f;
+}
+
 // lane j of column c's group writes entries j, 32 + j, ... of the column's row
 __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict__ col_ptr,
                                              int64_t ncol, const double* __restrict__ par,
+                                             const bf16* __restrict__ tails,
                                              const double* __restrict__ xv,
                                              const double* __restrict__ xi,
                                              const double* __restrict__ xe,
@@ -118,26 +137,38 @@ __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict_
         row[2 * kNodes + j] = feat(at_node(j, N, [&](int i) { return xi[b + i]; }) / nu);
         base = 3 * kNodes;
     }
+    // tail i in tail_feature's numbering (RichardsModel skips the heat flux)
     const int ntail = eh ? 9 : 8;
     for (int f = base + j; f < Fp; f += kGroup) {
         int i = f - base;
-        row[f] = i < ntail ? feat(tail_feature(i, eh, par[P_VG_ALPHA * ncol + c],
-                                               par[P_VG_N * ncol + c], par[P_KSAT * ncol + c], nu,
-                                               thr, ftw[c], eh ? fth[c] : 0.0, dt, N))
-                           : (bf16)0.0f;
+        if (!eh && i >= 6) ++i;
+        bf16 v = (bf16)0.0f;
+        if (f - base < ntail) {
+            if (i < 5) {
+                v = tails[c * kTails + i];
+            } else if (i == 8) {
+                v = tails[c * kTails + 5];
+            } else {  // forcing and step: one division, the same as tail_feature's
+                double num = i == 5 ? ftw[c] : (i == 6 ? fth[c] : dt);
+                double den = i == 5 ? 1e-5 : (i == 6 ? 100.0 : 3600.0);
+                v = feat(num / den);
+            }
+        }
+        row[f] = v;
     }
 }
 
 __global__ __launch_bounds__(256) void k_features(
     bool eh, const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
-    const double* __restrict__ xv, const double* __restrict__ xi, const double* __restrict__ xe,
-    const double* __restrict__ ftw, const double* __restrict__ fth, double dt, int Fp,
-    bf16* __restrict__ X) {
+    const bf16* __restrict__ tails, const double* __restrict__ xv, const double* __restrict__ xi,
+    const double* __restrict__ xe, const double* __restrict__ ftw, const double* __restrict__ fth,
+    double dt, int Fp, bf16* __restrict__ X) {
     int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t c = gt / kGroup;
     const int j = (int)(gt % kGroup);
     if (c >= ncol) return;
-    feature_lane(eh, col_ptr, ncol, par, xv, xi, xe, ftw, fth, dt, Fp, c, j, X + c * (int64_t)Fp);
+    feature_lane(eh, col_ptr, ncol, par, tails, xv, xi, xe, ftw, fth, dt, Fp, c, j,
+                 X + c * (int64_t)Fp);
 }
 
 // ---- training targets from (u_n, U0_ϑ, x_{n+1}): 32 lanes per column ----------
@@ -632,9 +663,9 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
 template <bool EH>
 __global__ __launch_bounds__(512, 1) void k_ws_infer(
     const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
-    const double* __restrict__ xv, const double* __restrict__ xi, const double* __restrict__ xe,
-    const double* __restrict__ ftw, const double* __restrict__ fth, double dt,
-    const bf16* __restrict__ W0, const bf16* __restrict__ W1, const bf16* __restrict__ W2,
+    const bf16* __restrict__ tails, const double* __restrict__ xv, const double* __restrict__ xi,
+    const double* __restrict__ xe, const double* __restrict__ ftw, const double* __restrict__ fth,
+    double dt, const bf16* __restrict__ W0, const bf16* __restrict__ W1, const bf16* __restrict__ W2,
     const bf16* __restrict__ W3, const float* __restrict__ b0, const float* __restrict__ b1,
     const float* __restrict__ b2, const float* __restrict__ b3, double* __restrict__ gv,
     double* __restrict__ ge) {
@@ -655,8 +686,8 @@ __global__ __launch_bounds__(512, 1) void k_ws_infer(
 #pragma unroll 4
         for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
             const int64_t c = c0 + lc < ncol ? c0 + lc : ncol - 1;
-            feature_lane(EH, col_ptr, ncol, par, xv, xi, xe, ftw, fth, dt, D::Fp, c, lane & 31,
-                         Act + lc * LR);
+            feature_lane(EH, col_ptr, ncol, par, tails, xv, xi, xe, ftw, fth, dt, D::Fp, c,
+                         lane & 31, Act + lc * LR);
         }
         first.store(Wb, 0);
     }

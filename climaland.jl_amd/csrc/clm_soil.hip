@@ -480,6 +480,7 @@ struct WsState {
     float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
     bf16 *wt = nullptr, *wn = nullptr;  // BF16 Wt [N x K] and W = Wt^T [K x N]
     bf16* wpk = nullptr;                // BF16 Wt chunk-major [K/32][N][32] (fused inference)
+    bf16* tails = nullptr;              // per-column static tail features [ncol][8]
     int64_t adam_t = 0;
     bool enabled = false;
     // inference (one row per column)
@@ -531,6 +532,7 @@ void ws_free(WsState* w) {
     dfree(w->wt);
     dfree(w->wn);
     dfree(w->wpk);
+    dfree(w->tails);
     dfree(w->X);
     dfree(w->Ha);
     dfree(w->Hb);
@@ -673,8 +675,8 @@ int ws_batch_buffers(clm_ctx* c, WsState* w, int64_t M) {
 int ws_features_launch(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
                        bf16* X) {
     hipLaunchKernelGGL(ws::k_features, grid1(c->ncol * ws::kGroup), dim3(256), 0, c->stream, is_eh(c),
-                       c->d_col_ptr, c->ncol, c->d_cpar, c->d_cv, c->d_ci, c->d_ce, ftw,
-                       fth ? fth : ftw, dt, w->d.Fp, X);
+                       c->d_col_ptr, c->ncol, c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce,
+                       ftw, fth ? fth : ftw, dt, w->d.Fp, X);
     return CLM_OK;
 }
 
@@ -689,12 +691,13 @@ void ws_guess(clm_ctx* c, WsState* w, const double* ftw, const double* fth, doub
         dim3 g((unsigned)((c->ncol + ws::kIBM - 1) / ws::kIBM)), b(512);
         if (is_eh(c))
             hipLaunchKernelGGL(ws::k_ws_infer<true>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
-                               c->d_cpar, c->d_cv, c->d_ci, c->d_ce, ftw, fth ? fth : ftw, dt,
+                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw,
+                               fth ? fth : ftw, dt,
                                W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
                                bias(0), bias(1), bias(2), bias(3), w->gv, w->ge);
         else
             hipLaunchKernelGGL(ws::k_ws_infer<false>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
-                               c->d_cpar, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
+                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
                                W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
                                bias(0), bias(1), bias(2), bias(3), w->gv, w->ge);
         return;
@@ -1685,7 +1688,7 @@ int clm_ws_init(clm_ctx* c, uint64_t seed) {
     if ((r = dalloc(c, &w->p, w->P)) || (r = dalloc(c, &w->g, w->P)) ||
         (r = dalloc(c, &w->m1, w->P)) || (r = dalloc(c, &w->m2, w->P)) ||
         (r = dalloc(c, &w->wt, w->PW)) || (r = dalloc(c, &w->wn, w->PW)) ||
-        (r = dalloc(c, &w->wpk, w->PW)) ||
+        (r = dalloc(c, &w->wpk, w->PW)) || (r = dalloc(c, &w->tails, c->ncol * ws::kTails)) ||
         (r = dalloc(c, &w->X, c->ncol * w->d.Fp)) || (r = dalloc(c, &w->Ha, c->ncol * H)) ||
         (r = dalloc(c, &w->Hb, c->ncol * H)) || (r = dalloc(c, &w->Y, c->ncol * w->d.Nout)) ||
         (r = dalloc(c, &w->gv, c->ncell)) || (r = dalloc(c, &w->ge, c->ncell)) ||
@@ -1712,6 +1715,8 @@ int clm_ws_init(clm_ctx* c, uint64_t seed) {
     }
     hipError_t e = hipMemcpyAsync(w->p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
+        hipLaunchKernelGGL(ws::k_static_tails, grid1(c->ncol), dim3(256), 0, c->stream, is_eh(c),
+                           c->d_col_ptr, c->ncol, c->d_cpar, w->tails);
         ws_cast(c, w);
         e = hipStreamSynchronize(c->stream);
     }
