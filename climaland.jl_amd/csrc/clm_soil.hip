@@ -83,6 +83,7 @@ struct clm_ctx {
         int G, R;
         int32_t n;
         int32_t* d_cols;
+        int32_t* d_tmp = nullptr;  // scratch of the regrouping kernel
     };
     std::vector<Bin> bins;  // one per depth bucket (n may be 0)
     int bucket_G[16], bucket_R[16], bucket_nmax[16];
@@ -446,6 +447,75 @@ void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
     for (const auto& bn : c->bins) launch_bin_any<JT>(c, bn, ftw, fth, dt, iters, x);
 }
 
+// ---- regrouping of a bin's columns by Newton iteration count ---------------------
+// The lanes of a wave serve 64 / G columns and the wave runs until its slowest
+// column has converged.  After each step every bin's column list is reordered
+// by that step's iteration counts, largest first (counting sort, 64 buckets,
+// ties in any order), so that the next step packs columns of similar counts
+// into a wave; counts change slowly from step to step.  Each column's
+// arithmetic does not depend on its neighbours, so results are unchanged.
+constexpr int kRgBuckets = 64, kRgWaves = 16, kRgMaxBins = 16;
+struct RegroupArgs {
+    int32_t* cols[kRgMaxBins];
+    int32_t* tmp[kRgMaxBins];
+    int32_t n[kRgMaxBins];
+};
+
+// one 1024-thread block per bin; wave w sorts the contiguous segment w of it
+__global__ __launch_bounds__(1024) void k_regroup(RegroupArgs a, const int32_t* __restrict__ iters) {
+    __shared__ int cnt[kRgWaves][kRgBuckets];
+    int32_t* __restrict__ cols = a.cols[blockIdx.x];
+    int32_t* __restrict__ tmp = a.tmp[blockIdx.x];
+    const int n = a.n[blockIdx.x];
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    for (int i = t; i < kRgWaves * kRgBuckets; i += 1024) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+    const int per = (n + kRgWaves - 1) / kRgWaves;
+    const int lo = min(n, w * per), hi = min(n, lo + per);
+    auto bucket = [&](int32_t col) {
+        return kRgBuckets - 1 - min(iters[col] & 0xFFFF, kRgBuckets - 1);
+    };
+    for (int i = lo + lane; i < hi; i += 64) {
+        int32_t col = cols[i];
+        tmp[i] = col;
+        atomicAdd(&cnt[w][bucket(col)], 1);
+    }
+    __syncthreads();
+    if (w == 0) {  // lane q owns bucket q: offsets bucket-major, then by wave
+        int tot = 0;
+        for (int v = 0; v < kRgWaves; ++v) tot += cnt[v][lane];
+        int incl = tot;
+        for (int o = 1; o < kRgBuckets; o <<= 1) {
+            int y = __shfl_up(incl, (unsigned)o, kRgBuckets);
+            if (lane >= o) incl += y;
+        }
+        int run = incl - tot;
+        for (int v = 0; v < kRgWaves; ++v) {
+            int k = cnt[v][lane];
+            cnt[v][lane] = run;
+            run += k;
+        }
+    }
+    __syncthreads();
+    for (int i = lo + lane; i < hi; i += 64) {
+        int32_t col = tmp[i];  // written by this thread above
+        cols[atomicAdd(&cnt[w][bucket(col)], 1)] = col;
+    }
+}
+
+void launch_regroup(clm_ctx* c, const int32_t* iters) {
+    RegroupArgs a{};
+    int nb = 0;
+    for (const auto& bn : c->bins)
+        if (bn.n > 0 && nb < kRgMaxBins) {
+            a.cols[nb] = bn.d_cols;
+            a.tmp[nb] = bn.d_tmp;
+            a.n[nb] = bn.n;
+            ++nb;
+        }
+    if (nb > 0) hipLaunchKernelGGL(k_regroup, dim3((unsigned)nb), dim3(1024), 0, c->stream, a, iters);
+}
+
 void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
                      StepExtra* x);
 
@@ -459,6 +529,7 @@ void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, in
             launch_group_t<float>(c, ftw, fth, dt, iters, x);
         else
             launch_group_t<double>(c, ftw, fth, dt, iters, x);
+        launch_regroup(c, iters);
         return;
     }
     if (c->jac_prec == 32)
@@ -946,6 +1017,7 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
             clm_ctx::Bin bn{c->bucket_G[b], c->bucket_R[b], (int32_t)cols[b].size(), nullptr};
             if (bn.n > 0) {
                 ALLOC(bn.d_cols, bn.n);
+                ALLOC(bn.d_tmp, bn.n);
                 if (hipMemcpyAsync(bn.d_cols, cols[b].data(), 4 * cols[b].size(),
                                    hipMemcpyHostToDevice, c->stream) != hipSuccess ||
                     hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -1082,7 +1154,10 @@ int clm_destroy(clm_ctx* c) {
     dfree(c->d_cfth);
     dfree(c->d_cfbw);
     dfree(c->d_cfbh);
-    for (auto& bn : c->bins) dfree(bn.d_cols);
+    for (auto& bn : c->bins) {
+        dfree(bn.d_cols);
+        dfree(bn.d_tmp);
+    }
     ws_free(c->ws);
     c->ws = nullptr;
     if (c->ev0) hipEventDestroy(c->ev0);
