@@ -82,9 +82,12 @@ struct clm_ctx {
     struct Bin {
         int G, R;
         int32_t n;
-        int32_t* d_cols;
-        int32_t* d_tmp = nullptr;  // scratch of the regrouping kernel
+        int32_t* d_cols;  // into d_bincols
     };
+    // all bins' column lists back to back (bin b at bin_off[b]), a copy of
+    // them, and the regrouping histogram / fill pointers [bin][bucket]
+    int32_t *d_bincols = nullptr, *d_bintmp = nullptr, *d_rghist = nullptr, *d_rgfill = nullptr;
+    int32_t bin_off[17] = {0};
     std::vector<Bin> bins;  // one per depth bucket (n may be 0)
     int bucket_G[16], bucket_R[16], bucket_nmax[16];
     struct WsState* ws = nullptr;  // learned warm start (clm_ws_*)
@@ -164,6 +167,7 @@ constexpr int kNMenu = sizeof(kMenu) / sizeof(kMenu[0]);
 // depth buckets: columns with kBucketMax[b-1] < N <= kBucketMax[b] share a shape
 constexpr int kBucketMax[] = {8, 16, 24, 32, 40, 48, 64, 80, 96, 128};
 constexpr int kNBucket = sizeof(kBucketMax) / sizeof(kBucketMax[0]);
+constexpr int kRgBuckets = 64;  // iteration-count buckets of the bin regrouping
 // shapes before autotuning: few lanes per column, several rows per lane
 constexpr Shape kDefaultShape[kNBucket] = {{8, 1},  {8, 2},  {8, 3},  {8, 4},  {16, 3},
                                            {16, 3}, {16, 4}, {32, 3}, {32, 3}, {32, 4}};
@@ -450,70 +454,93 @@ void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
 // ---- regrouping of a bin's columns by Newton iteration count ---------------------
 // The lanes of a wave serve 64 / G columns and the wave runs until its slowest
 // column has converged.  After each step every bin's column list is reordered
-// by that step's iteration counts, largest first (counting sort, 64 buckets,
-// ties in any order), so that the next step packs columns of similar counts
-// into a wave; counts change slowly from step to step.  Each column's
+// by that step's iteration counts, largest first (a counting sort over 64
+// buckets, ties in any order), so that the next step packs columns of similar
+// counts into a wave; counts change slowly from step to step.  Each column's
 // arithmetic does not depend on its neighbours, so results are unchanged.
-constexpr int kRgBuckets = 64, kRgWaves = 16, kRgMaxBins = 16;
-struct RegroupArgs {
-    int32_t* cols[kRgMaxBins];
-    int32_t* tmp[kRgMaxBins];
-    int32_t n[kRgMaxBins];
+// Three launches over all bins' lists at once (back to back in one array):
+// count (block histograms, then one global atomic per bin and bucket), scan
+// (fill pointers; clears the histogram for the next step) and scatter.
+struct BinOffsets {
+    int32_t off[kNBucket + 1];
 };
+__device__ __forceinline__ int bin_of_index(const BinOffsets& bo, int i) {
+    int b = 0;
+#pragma unroll
+    for (int k = 1; k < kNBucket; ++k) b += i >= bo.off[k] ? 1 : 0;
+    return b;
+}
+__device__ __forceinline__ int rg_bucket(const int32_t* __restrict__ iters, int32_t col) {
+    return kRgBuckets - 1 - min(iters[col] & 0xFFFF, kRgBuckets - 1);
+}
 
-// one 1024-thread block per bin; wave w sorts the contiguous segment w of it
-__global__ __launch_bounds__(1024) void k_regroup(RegroupArgs a, const int32_t* __restrict__ iters) {
-    __shared__ int cnt[kRgWaves][kRgBuckets];
-    int32_t* __restrict__ cols = a.cols[blockIdx.x];
-    int32_t* __restrict__ tmp = a.tmp[blockIdx.x];
-    const int n = a.n[blockIdx.x];
-    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    for (int i = t; i < kRgWaves * kRgBuckets; i += 1024) (&cnt[0][0])[i] = 0;
+__global__ __launch_bounds__(256) void k_rg_count(const int32_t* __restrict__ cols,
+                                                  int32_t* __restrict__ tmp, int n, BinOffsets bo,
+                                                  const int32_t* __restrict__ iters,
+                                                  int32_t* __restrict__ hist) {
+    __shared__ int h[kNBucket * kRgBuckets];
+    for (int k = threadIdx.x; k < kNBucket * kRgBuckets; k += 256) h[k] = 0;
     __syncthreads();
-    const int per = (n + kRgWaves - 1) / kRgWaves;
-    const int lo = min(n, w * per), hi = min(n, lo + per);
-    auto bucket = [&](int32_t col) {
-        return kRgBuckets - 1 - min(iters[col] & 0xFFFF, kRgBuckets - 1);
-    };
-    for (int i = lo + lane; i < hi; i += 64) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
         int32_t col = cols[i];
         tmp[i] = col;
-        atomicAdd(&cnt[w][bucket(col)], 1);
+        atomicAdd(&h[bin_of_index(bo, i) * kRgBuckets + rg_bucket(iters, col)], 1);
     }
     __syncthreads();
-    if (w == 0) {  // lane q owns bucket q: offsets bucket-major, then by wave
-        int tot = 0;
-        for (int v = 0; v < kRgWaves; ++v) tot += cnt[v][lane];
-        int incl = tot;
-        for (int o = 1; o < kRgBuckets; o <<= 1) {
-            int y = __shfl_up(incl, (unsigned)o, kRgBuckets);
-            if (lane >= o) incl += y;
-        }
-        int run = incl - tot;
-        for (int v = 0; v < kRgWaves; ++v) {
-            int k = cnt[v][lane];
-            cnt[v][lane] = run;
-            run += k;
-        }
+    for (int k = threadIdx.x; k < kNBucket * kRgBuckets; k += 256)
+        if (h[k]) atomicAdd(&hist[k], h[k]);
+}
+
+// one wave per bin: lane q owns bucket q
+__global__ __launch_bounds__(64 * kNBucket) void k_rg_scan(BinOffsets bo, int32_t* __restrict__ hist,
+                                                           int32_t* __restrict__ fill) {
+    const int b = threadIdx.x >> 6, q = threadIdx.x & 63;
+    const int k = b * kRgBuckets + q;
+    const int cnt = hist[k];
+    int incl = cnt;
+    for (int o = 1; o < kRgBuckets; o <<= 1) {
+        int y = __shfl_up(incl, (unsigned)o, 64);
+        if (q >= o) incl += y;
+    }
+    fill[k] = bo.off[b] + incl - cnt;
+    hist[k] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_rg_scatter(int32_t* __restrict__ cols,
+                                                    const int32_t* __restrict__ tmp, int n,
+                                                    BinOffsets bo,
+                                                    const int32_t* __restrict__ iters,
+                                                    int32_t* __restrict__ fill) {
+    __shared__ int h[kNBucket * kRgBuckets];
+    for (int k = threadIdx.x; k < kNBucket * kRgBuckets; k += 256) h[k] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int32_t col = 0;
+    int key = 0, rank = 0;
+    if (i < n) {
+        col = tmp[i];
+        key = bin_of_index(bo, i) * kRgBuckets + rg_bucket(iters, col);
+        rank = atomicAdd(&h[key], 1);
     }
     __syncthreads();
-    for (int i = lo + lane; i < hi; i += 64) {
-        int32_t col = tmp[i];  // written by this thread above
-        cols[atomicAdd(&cnt[w][bucket(col)], 1)] = col;
-    }
+    for (int k = threadIdx.x; k < kNBucket * kRgBuckets; k += 256)
+        if (h[k]) h[k] = atomicAdd(&fill[k], h[k]);  // this block's base in bucket k
+    __syncthreads();
+    if (i < n) cols[h[key] + rank] = col;
 }
 
 void launch_regroup(clm_ctx* c, const int32_t* iters) {
-    RegroupArgs a{};
-    int nb = 0;
-    for (const auto& bn : c->bins)
-        if (bn.n > 0 && nb < kRgMaxBins) {
-            a.cols[nb] = bn.d_cols;
-            a.tmp[nb] = bn.d_tmp;
-            a.n[nb] = bn.n;
-            ++nb;
-        }
-    if (nb > 0) hipLaunchKernelGGL(k_regroup, dim3((unsigned)nb), dim3(1024), 0, c->stream, a, iters);
+    BinOffsets bo;
+    for (int b = 0; b <= kNBucket; ++b) bo.off[b] = c->bin_off[b];
+    const int n = (int)c->ncol;
+    const dim3 g((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_rg_count, g, dim3(256), 0, c->stream, c->d_bincols, c->d_bintmp, n, bo,
+                       iters, c->d_rghist);
+    hipLaunchKernelGGL(k_rg_scan, dim3(1), dim3(64 * kNBucket), 0, c->stream, bo, c->d_rghist,
+                       c->d_rgfill);
+    hipLaunchKernelGGL(k_rg_scatter, g, dim3(256), 0, c->stream, c->d_bincols, c->d_bintmp, n, bo,
+                       iters, c->d_rgfill);
 }
 
 void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
@@ -1015,17 +1042,25 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
             c->bucket_G[b] = kDefaultShape[b].G;
             c->bucket_R[b] = kDefaultShape[b].R;
             clm_ctx::Bin bn{c->bucket_G[b], c->bucket_R[b], (int32_t)cols[b].size(), nullptr};
-            if (bn.n > 0) {
-                ALLOC(bn.d_cols, bn.n);
-                ALLOC(bn.d_tmp, bn.n);
-                if (hipMemcpyAsync(bn.d_cols, cols[b].data(), 4 * cols[b].size(),
-                                   hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-                    hipStreamSynchronize(c->stream) != hipSuccess) {
-                    set_err(c, CLM_E_HIP, "bin upload failed");
-                    return fail(CLM_E_HIP);
-                }
-            }
+            c->bin_off[b + 1] = c->bin_off[b] + bn.n;
             c->bins.push_back(bn);
+        }
+        ALLOC(c->d_bincols, c->ncol);
+        ALLOC(c->d_bintmp, c->ncol);
+        ALLOC(c->d_rghist, kNBucket * kRgBuckets);
+        ALLOC(c->d_rgfill, kNBucket * kRgBuckets);
+        std::vector<int32_t> all;
+        all.reserve((size_t)c->ncol);
+        for (int b = 0; b < kNBucket; ++b) {
+            c->bins[b].d_cols = c->d_bincols + c->bin_off[b];
+            all.insert(all.end(), cols[b].begin(), cols[b].end());
+        }
+        if (hipMemcpyAsync(c->d_bincols, all.data(), 4 * all.size(), hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess ||
+            hipMemsetAsync(c->d_rghist, 0, 4 * kNBucket * kRgBuckets, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            set_err(c, CLM_E_HIP, "bin upload failed");
+            return fail(CLM_E_HIP);
         }
     }
 #undef ALLOC
@@ -1154,10 +1189,10 @@ int clm_destroy(clm_ctx* c) {
     dfree(c->d_cfth);
     dfree(c->d_cfbw);
     dfree(c->d_cfbh);
-    for (auto& bn : c->bins) {
-        dfree(bn.d_cols);
-        dfree(bn.d_tmp);
-    }
+    dfree(c->d_bincols);
+    dfree(c->d_bintmp);
+    dfree(c->d_rghist);
+    dfree(c->d_rgfill);
     ws_free(c->ws);
     c->ws = nullptr;
     if (c->ev0) hipEventDestroy(c->ev0);
