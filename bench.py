@@ -33,7 +33,8 @@ HBM_PEAK_GBS = 8000.0
 # and WRITE_SIZE summed over the bins' launches, FETCH scaled x2.0 and WRITE
 # x1.0 by a plain 1 GiB device copy); raw sums beside them.  Summary and
 # commands: profiles/k_eh_coop_eh_262144_uneven.md.
-MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 873.0e6, "raw": 531.3e6}}
+MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1218.6e6, "raw": 709.3e6},
+                    ("eh_262144_uneven", "mixed"): {"scaled": 1049.3e6, "raw": 624.7e6}}
 
 
 def step_bytes(batch, learned: bool) -> float:
