@@ -258,6 +258,49 @@ def _per_step_parity(name, ncol, jac, max_iters, tol, nsteps, keys, label, solve
     assert nwell >= min_well * nconv
 
 
+def _custom_workload(base, name, **kw):
+    import dataclasses
+    from clmsoil import inputs
+    inputs.WORKLOADS[name] = dataclasses.replace(inputs.WORKLOADS[base], name=name, **kw)
+    return inputs.WORKLOADS
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+@pytest.mark.parametrize("eh", [False, True], ids=["rre", "eh"])
+def test_shallowest_columns(gpu_lib, solver, eh):
+    """Columns of 2-8 layers (the smallest depth bucket; two layers is the
+    minimum clm_create accepts): every cell is a boundary cell or next to one."""
+    base = "eh_65536x20" if eh else "rre_131072_uneven"
+    keys = ("vartheta_l", "theta_i", "rho_e_int") if eh else ("vartheta_l", "intF_water")
+    W = _custom_workload(base, "shallow_test", config_id=11, layers=(2, 8), depth=(0.2, 2.0),
+                         ncol=256)
+    try:
+        _per_step_parity("shallow_test", 256, 64, 50, 1e-8, 12, keys,
+                         f"{'eh' if eh else 'rre'} shallow s{solver}", solver, min_well=0.75)
+    finally:
+        del W["shallow_test"]
+
+
+def test_rre_deeper_than_the_group_solver(gpu_lib):
+    """RichardsModel columns of 129-600 layers, beyond the group solver's 128:
+    the default solver choice falls back to one column per lane."""
+    W = _custom_workload("rre_131072_uneven", "deep_lane_test", config_id=12,
+                         layers=(129, 600), depth=(5.0, 20.0), ncol=64)
+    try:
+        _per_step_parity("deep_lane_test", 64, 64, 50, 1e-8, 6, ("vartheta_l", "intF_water"),
+                         "rre deep lane", 0, min_well=0.5)
+    finally:
+        del W["deep_lane_test"]
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_single_column(gpu_lib, solver):
+    """A batch of one EnergyHydrology column (one partial wave, one bin)."""
+    _per_step_parity("eh_262144_uneven", 1, 64, 50, 1e-8, 24,
+                     ("vartheta_l", "theta_i", "rho_e_int"), f"eh single s{solver}", solver,
+                     min_well=0.0)
+
+
 @pytest.mark.parametrize("solver", SOLVERS)
 def test_rre_per_step_fixed_iters(gpu_lib, solver):
     """rre_4096x50 semantics: 3 fixed Newton iterations, all 96 steps.
