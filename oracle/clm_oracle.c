@@ -329,6 +329,10 @@ static void load_par(const orc_model* m, int64_t c, col_par* q) {
         q->f_max = P[ORC_P_F_MAX * nc + c];
         q->f_over = P[ORC_P_F_OVER * nc + c];
         q->R_sb = P[ORC_P_R_SB * nc + c];
+    } else if (m->bc_top == ORC_BC_ATMOS_TOPMODEL) {
+        q->f_max = P[ORC_P_EH_F_MAX * nc + c];
+        q->f_over = P[ORC_P_EH_F_OVER * nc + c];
+        q->R_sb = P[ORC_P_EH_R_SB * nc + c];
     }
 }
 
@@ -339,9 +343,14 @@ typedef struct {
 } col_state;
 
 typedef struct {
-    double *K, *psi, *thl, *T, *kap, *sink; /* cells */
-    double *tw, *th, *bw, *bh, *rss;        /* scalars */
+    double *K, *psi, *thl, *T, *kap, *sink, *sink_e; /* cells */
+    double *tw, *th, *bw, *bh, *rss, *ress, *vice;   /* scalars */
 } col_aux;
+
+/* AtmosDrivenFluxBC inputs of one column */
+typedef struct {
+    double P_liq, T_air, lhf, shf, vliq, vice, R_n;
+} sfc_in;
 
 /* ---- RichardsModel ---------------------------------------------------- */
 
@@ -559,6 +568,70 @@ static void eh_bc(int bc_bottom, const double* K, const double* T, double fw, do
     }
 }
 
+/* TOPMODEL for EnergyHydrology (Runoff.jl:234-290, 373-416): the surface
+ * pass weights layers by total water (ice included), with the infiltration
+ * capacity -K_sat F(θ_i/(θ_l+θ_i-θ_r)) G(T) of the top layer; the subsurface
+ * pass by liquid water only, and R_ess carries the liquid energy of the
+ * saturated layers: R_ess = R_ss/max(h∇,eps) Σ w ρe_l(T) Δz.  The sources
+ * (Runoff.jl:346-359) remove R_ss/max(h∇,eps) w from ϑ_l, R_ess/max(h∇,eps) w
+ * from ρe_int, R_ss from ∫F_w and R_ess from ∫F_e. */
+static void eh_topmodel(const col_geom* g, const col_par* q, const double* vl, const double* ti,
+                        const double* thl, const double* T, double P, double* infil,
+                        double* rss, double* ress, double* sink_v, double* sink_e) {
+    const int N = g->N;
+    const double nsat = q->nu - q->theta_r;
+    double hs = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double twc = vl[i] + ti[i] - q->theta_r;
+        hs += orc_heaviside(twc - nsat) * twc / nsat * dzc(g, i);
+    }
+    const double depth = g->zf[N] - g->zf[0];
+    const int t = N - 1;
+    double ic = -q->Ksat * orc_impedance_factor(ti[t] / (thl[t] + ti[t] - q->theta_r), OMEGA) *
+                orc_viscosity_factor(T[t], GAMMA_VISC, GAMMA_T_REF);
+    double fsat = fmin(q->f_max * EXP(-q->f_over / 2.0 * (depth - hs)), 1.0);
+    *infil = (1.0 - fsat) * fmax(ic, P);
+    double h = 0.0, he = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double twc = vl[i] - q->theta_r;
+        double w = orc_heaviside(twc - nsat) * twc / nsat;
+        sink_v[i] = w;
+        h += w * dzc(g, i);
+        he += w * orc_volumetric_internal_energy_liq(T[i]) * dzc(g, i);
+    }
+    *rss = q->R_sb * EXP(-q->f_over * (depth - h));
+    *ress = he * (*rss / fmax(h, DBL_EPSILON));
+    for (int i = 0; i < N; ++i) {
+        sink_e[i] = (*ress / fmax(h, DBL_EPSILON)) * sink_v[i];
+        sink_v[i] = (*rss / fmax(h, DBL_EPSILON)) * sink_v[i];
+    }
+}
+
+/* AtmosDrivenFluxBC top fluxes in soil-only form (boundary_conditions.jl:880-
+ * 915, 955-1003): infiltration (NoRunoff: the precipitation; TOPMODEL above)
+ * plus the liquid vapour flux, and R_n + lhf + shf + infiltration ρe_l(T_air). */
+static void eh_atmos_top(const orc_model* m, const col_geom* g, const col_par* q,
+                         const double* vl, const double* ti, const double* thl, const double* T,
+                         const sfc_in* s, double* tw, double* th, double* rss, double* ress,
+                         double* sink_v, double* sink_e) {
+    double infil = s->P_liq;
+    *rss = *ress = 0.0;
+    if (m->bc_top == ORC_BC_ATMOS_TOPMODEL)
+        eh_topmodel(g, q, vl, ti, thl, T, s->P_liq, &infil, rss, ress, sink_v, sink_e);
+    *tw = infil + s->vliq;
+    *th = s->R_n + s->lhf + s->shf + infil * orc_volumetric_internal_energy_liq(s->T_air);
+}
+
+/* SoilSublimation source! (energy_hydrology.jl:931-945), explicit: the top
+ * layer (H(z + 2 Δz_top), z the layer centre) loses ρ_l/ρ_i E_ice/(2 Δz_top)
+ * of ice, ∫F_w the flux E_ice */
+static void eh_sublimation(const col_geom* g, double vice, double* dti, double* diw) {
+    const double dzt = dztop(g);
+    for (int i = 0; i < g->N; ++i)
+        dti[i] += -vice * RHO_L / RHO_I * orc_heaviside(zc(g, i) + 2.0 * dzt) / (2.0 * dzt);
+    *diw += -vice;
+}
+
 /* energy_hydrology.jl:363-425 compute_imp_tendency */
 static void eh_imp_tendency(const col_geom* g, const double* K, const double* psi,
                             const double* T, const double* kap, double tw, double th,
@@ -591,6 +664,17 @@ static void eh_imp_tendency(const col_geom* g, const double* K, const double* ps
         Flo = Fhi;
         Glo = Ghi;
     }
+}
+
+/* implicit TOPMODELSubsurfaceRunoff source on EH (Runoff.jl:346-359) */
+static void eh_runoff_sink(int N, const double* sink_v, const double* sink_e, double rss,
+                           double ress, double* dvl, double* dre, double* diw, double* die) {
+    for (int i = 0; i < N; ++i) {
+        dvl[i] -= sink_v[i];
+        dre[i] -= sink_e[i];
+    }
+    *diw -= rss;
+    *die -= ress;
 }
 
 /* energy_hydrology.jl:838-906 PhaseChange source! on zeroed dY
@@ -739,16 +823,18 @@ static int rre_step_col(const orc_model* m, int64_t c, col_state* Y, double f_to
 }
 
 static int eh_step_col(const orc_model* m, int64_t c, col_state* Y, double f_top_w,
-                       double f_top_h, double f_bot_w, double f_bot_h, double dt,
-                       int max_iters, double tol, int* iters, col_scratch* S, int* perr,
-                       const double* gv, const double* ge) {
+                       double f_top_h, double f_bot_w, double f_bot_h, const sfc_in* sf,
+                       double dt, int max_iters, double tol, int* iters, col_scratch* S,
+                       int* perr, const double* gv, const double* ge) {
     int32_t a = m->col_ptr[c];
     int N = m->col_ptr[c + 1] - a;
     col_geom g = {N, m->z_face + a + c};
     col_par q;
     load_par(m, c, &q);
-    double* w = scratch_get(S, 40 * N + 8);
+    double* w = scratch_get(S, 44 * N + 8);
     double *K = w, *psi = w + N, *thl = w + 2 * N, *T = w + 3 * N, *kap = w + 4 * N;
+    double *sv = w + 40 * N, *se = w + 41 * N, rss = 0.0, ress = 0.0;
+    const int topm = m->bc_top == ORC_BC_ATMOS_TOPMODEL;
     double *U0v = w + 5 * N, *U0i = w + 6 * N, *U0e = w + 7 * N;
     double *Rv = w + 8 * N, *Ri = w + 9 * N, *Re = w + 10 * N;
     double *dv = w + 11 * N, *di = w + 12 * N, *de = w + 13 * N;
@@ -758,8 +844,10 @@ static int eh_step_col(const orc_model* m, int64_t c, col_state* Y, double f_top
     /* stage 1: update_cache!(p, u_n) then exp tendency (phase change) */
     eh_aux(&g, &q, Y->vl, Y->ti, Y->re, thl, kap, T, K, psi);
     eh_bc(m->bc_bottom, K, T, f_top_w, f_top_h, f_bot_w, f_bot_h, &tw, &th, &bw, &bh);
+    if (sf) eh_atmos_top(m, &g, &q, Y->vl, Y->ti, thl, T, sf, &tw, &th, &rss, &ress, sv, se);
     double dexw, dexe;
     *perr |= eh_exp_tendency(&g, &q, Y->ti, thl, T, kap, Rv, Ri, Re, &dexw, &dexe);
+    if (sf) eh_sublimation(&g, sf->vice, Ri, &dexw);
     for (int i = 0; i < N; ++i) {
         U0v[i] = Y->vl[i] + dt * Rv[i];
         U0i[i] = Y->ti[i] + dt * Ri[i];
@@ -783,6 +871,7 @@ static int eh_step_col(const orc_model* m, int64_t c, col_state* Y, double f_top
         eh_jacobian(&g, &q, Y->vl, Y->ti, K, T, kap, thl, dt, Www, Wee, Wew, tmpD, tmpK);
         double diw, die;
         eh_imp_tendency(&g, K, psi, T, kap, tw, th, bw, bh, Rv, Ri, Re, &diw, &die);
+        if (topm) eh_runoff_sink(N, sv, se, rss, ress, Rv, Re, &diw, &die);
         for (int i = 0; i < N; ++i) {
             Rv[i] = U0v[i] + dt * Rv[i] - Y->vl[i];
             Ri[i] = U0i[i] + dt * Ri[i] - Y->ti[i];
@@ -834,8 +923,13 @@ static void view_aux(const orc_model* m, const orc_aux* p, int64_t c, col_aux* v
     v->psi = p->psi + a;
     v->tw = p->top_water + c;
     v->bw = p->bot_water + c;
-    v->sink = (m->bc_top == ORC_BC_TOPMODEL && p->sink) ? p->sink + a : NULL;
-    v->rss = (m->bc_top == ORC_BC_TOPMODEL && p->r_ss) ? p->r_ss + c : NULL;
+    const int topm = m->bc_top == ORC_BC_TOPMODEL || m->bc_top == ORC_BC_ATMOS_TOPMODEL;
+    const int atm = m->bc_top == ORC_BC_ATMOS || m->bc_top == ORC_BC_ATMOS_TOPMODEL;
+    v->sink = (topm && p->sink) ? p->sink + a : NULL;
+    v->rss = (topm && p->r_ss) ? p->r_ss + c : NULL;
+    v->sink_e = (m->bc_top == ORC_BC_ATMOS_TOPMODEL && p->sink_e) ? p->sink_e + a : NULL;
+    v->ress = (m->bc_top == ORC_BC_ATMOS_TOPMODEL && p->r_ess) ? p->r_ess + c : NULL;
+    v->vice = (atm && p->vapor_ice) ? p->vapor_ice + c : NULL;
     if (m->model == ORC_MODEL_ENERGY_HYDROLOGY) {
         v->thl = p->theta_l + a;
         v->T = p->T + a;
@@ -848,6 +942,15 @@ static void view_aux(const orc_model* m, const orc_aux* p, int64_t c, col_aux* v
 }
 
 static inline double fval(const double* a, int64_t c) { return a ? a[c] : 0.0; }
+
+static inline int is_atmos(const orc_model* m) {
+    return m->bc_top == ORC_BC_ATMOS || m->bc_top == ORC_BC_ATMOS_TOPMODEL;
+}
+static sfc_in sfc_of(const orc_forcing* f, int64_t c) {
+    sfc_in s = {fval(f->P_liq, c), fval(f->T_air, c),     fval(f->lhf, c), fval(f->shf, c),
+                fval(f->vapor_liq, c), fval(f->vapor_ice, c), fval(f->R_n, c)};
+    return s;
+}
 
 int orc_update_cache(const orc_model* m, const orc_state* Y, const orc_forcing* f,
                      orc_aux* p) {
@@ -869,6 +972,15 @@ int orc_update_cache(const orc_model* m, const orc_state* Y, const orc_forcing* 
             eh_aux(&g, &q, y.vl, y.ti, y.re, a.thl, a.kap, a.T, a.K, a.psi);
             eh_bc(m->bc_bottom, a.K, a.T, fval(f->top_water, c), fval(f->top_heat, c),
                   fval(f->bot_water, c), fval(f->bot_heat, c), a.tw, a.th, a.bw, a.bh);
+            if (is_atmos(m)) {
+                sfc_in s = sfc_of(f, c);
+                double rss, ress;
+                eh_atmos_top(m, &g, &q, y.vl, y.ti, a.thl, a.T, &s, a.tw, a.th, &rss, &ress,
+                             a.sink, a.sink_e);
+                if (a.rss) *a.rss = rss;
+                if (a.ress) *a.ress = ress;
+                if (a.vice) *a.vice = s.vice;
+            }
         }
     }
     return 0;
@@ -917,6 +1029,7 @@ int orc_exp_tendency(const orc_model* m, const orc_state* Y, const orc_aux* p,
             *d.iw = 0.0;
         } else {
             err |= eh_exp_tendency(&g, &q, y.ti, a.thl, a.T, a.kap, d.vl, d.ti, d.re, d.iw, d.ie);
+            if (a.vice) eh_sublimation(&g, *a.vice, d.ti, d.iw);
         }
     }
     return err ? -3 : 0;
@@ -935,9 +1048,12 @@ int orc_imp_tendency(const orc_model* m, const orc_state* Y, const orc_aux* p,
         if (m->model == ORC_MODEL_RICHARDS)
             rre_imp_tendency(&g, a.K, a.psi, *a.tw, *a.bw, a.sink, a.rss ? *a.rss : 0.0, d.vl,
                              d.iw);
-        else
+        else {
             eh_imp_tendency(&g, a.K, a.psi, a.T, a.kap, *a.tw, *a.th, *a.bw, *a.bh, d.vl, d.ti,
                             d.re, d.iw, d.ie);
+            if (a.sink && a.sink_e)
+                eh_runoff_sink(g.N, a.sink, a.sink_e, *a.rss, *a.ress, d.vl, d.re, d.iw, d.ie);
+        }
     }
     (void)Y;
     return 0;
@@ -1026,10 +1142,13 @@ int64_t orc_step_guess(const orc_model* m, orc_state* Y, const orc_forcing* f, d
             if (m->model == ORC_MODEL_RICHARDS)
                 nonconv += rre_step_col(m, c, &y, fval(f->top_water, c), fval(f->bot_water, c),
                                         dt, max_iters, tol, &it, &S, gv);
-            else
+            else {
+                sfc_in sf = sfc_of(f, c);
                 nonconv += eh_step_col(m, c, &y, fval(f->top_water, c), fval(f->top_heat, c),
-                                       fval(f->bot_water, c), fval(f->bot_heat, c), dt,
-                                       max_iters, tol, &it, &S, &err, gv, ge);
+                                       fval(f->bot_water, c), fval(f->bot_heat, c),
+                                       is_atmos(m) ? &sf : NULL, dt, max_iters, tol, &it, &S,
+                                       &err, gv, ge);
+            }
             if (iters_out) iters_out[c] = it;
         }
         free(S.buf);
@@ -1067,8 +1186,8 @@ int64_t orc_run(const orc_model* m, orc_state* Y, const double* top_water_tab,
                     nonconv += rre_step_col(m, c, &y, fw, 0.0, dt, max_iters, tol, &it, &S,
                                             NULL);
                 else
-                    nonconv += eh_step_col(m, c, &y, fw, fh, 0.0, 0.0, dt, max_iters, tol, &it,
-                                           &S, &err, NULL, NULL);
+                    nonconv += eh_step_col(m, c, &y, fw, fh, 0.0, 0.0, NULL, dt, max_iters, tol,
+                                           &it, &S, &err, NULL, NULL);
                 if (iters_out) iters_out[(int64_t)s * m->ncol + c] = it;
             }
         }

@@ -41,6 +41,9 @@ enum {
 /* RichardsModel with a TOPMODEL top boundary takes three more rows */
 enum { ORC_P_F_MAX = 6, ORC_P_F_OVER = 7, ORC_P_R_SB = 8 };
 #define ORC_NPARAM_RRE_TOPMODEL 9
+/* EnergyHydrology with AtmosDrivenFluxBC and TOPMODELRunoff: three more rows */
+enum { ORC_P_EH_F_MAX = 13, ORC_P_EH_F_OVER = 14, ORC_P_EH_R_SB = 15 };
+#define ORC_NPARAM_EH_TOPMODEL 16
 
 enum { ORC_MODEL_RICHARDS = 0, ORC_MODEL_ENERGY_HYDROLOGY = 1 };
 /* Water boundary conditions.  Bottom: prescribed flux (WaterFluxBC /
@@ -56,7 +59,12 @@ enum { ORC_BC_FLUX = 0, ORC_BC_FREE_DRAINAGE = 1, ORC_BC_MOISTURE_STATE = 2,
        ORC_BC_TOPMODEL = 3,
        /* top only, RRE: RichardsAtmosDrivenFluxBC(precipitation, SurfaceRunoff)
         * (Runoff.jl:84-148); the forcing value is the precipitation rate */
-       ORC_BC_SURFACE_RUNOFF = 4 };
+       ORC_BC_SURFACE_RUNOFF = 4,
+       /* top only, EH: AtmosDrivenFluxBC in soil-only form
+        * (boundary_conditions.jl:646-1003, energy_hydrology.jl:909-945) with
+        * the surface fluxes supplied per column (orc_forcing's surface
+        * members), NoRunoff (5) or TOPMODELRunoff (6, rows ORC_P_EH_*) */
+       ORC_BC_ATMOS = 5, ORC_BC_ATMOS_TOPMODEL = 6 };
 
 typedef struct {
     int32_t model;
@@ -91,6 +99,10 @@ typedef struct {
     double* bot_heat;
     double* sink;      /* TOPMODEL subsurface runoff sink per cell (1/s), may be NULL */
     double* r_ss;      /* TOPMODEL subsurface runoff R_ss per column (m/s), may be NULL */
+    double* sink_e;    /* EH TOPMODEL energy sink per cell (W/m3), may be NULL */
+    double* r_ess;     /* EH TOPMODEL energy runoff R_ess per column (W/m2), may be NULL */
+    double* vapor_ice; /* EH AtmosDrivenFluxBC: turbulent_fluxes.vapor_flux_ice per column,
+                          read by the explicit sublimation source; may be NULL */
 } orc_aux;
 
 /* prescribed fluxes (per column), bottom ones used only for ORC_BC_FLUX */
@@ -99,6 +111,16 @@ typedef struct {
     const double* top_heat;
     const double* bot_water;
     const double* bot_heat;
+    /* ORC_BC_ATMOS*: p.drivers.P_liq (m/s), p.drivers.T (K), the turbulent
+     * fluxes lhf, shf (W/m2), vapor_flux_liq, vapor_flux_ice (m/s) and R_n
+     * (W/m2), per column */
+    const double* P_liq;
+    const double* T_air;
+    const double* lhf;
+    const double* shf;
+    const double* vapor_liq;
+    const double* vapor_ice;
+    const double* R_n;
 } orc_forcing;
 
 /* ---- physical constants and pointwise closures (exported for KATs) ---- */

@@ -32,6 +32,8 @@ class Model(C.Structure):
 
 
 BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE, BC_TOPMODEL, BC_SURFACE_RUNOFF = 0, 1, 2, 3, 4
+BC_ATMOS, BC_ATMOS_TOPMODEL = 5, 6
+SURFACE_FIELDS = ("P_liq", "T_air", "lhf", "shf", "vapor_liq", "vapor_ice", "R_n")
 
 
 class State(C.Structure):
@@ -42,11 +44,13 @@ class State(C.Structure):
 class Aux(C.Structure):
     _fields_ = [("K", _dp), ("psi", _dp), ("theta_l", _dp), ("T", _dp), ("kappa", _dp),
                 ("top_water", _dp), ("top_heat", _dp), ("bot_water", _dp), ("bot_heat", _dp),
-                ("sink", _dp), ("r_ss", _dp)]
+                ("sink", _dp), ("r_ss", _dp), ("sink_e", _dp), ("r_ess", _dp),
+                ("vapor_ice", _dp)]
 
 
 class Forcing(C.Structure):
-    _fields_ = [("top_water", _dp), ("top_heat", _dp), ("bot_water", _dp), ("bot_heat", _dp)]
+    _fields_ = [("top_water", _dp), ("top_heat", _dp), ("bot_water", _dp), ("bot_heat", _dp)] + \
+        [(k, _dp) for k in SURFACE_FIELDS]
 
 
 _libs = {}
@@ -168,22 +172,30 @@ class OracleModel:
                                                "intF_water", "intF_energy")))
 
     def new_aux(self) -> dict:
-        a = {k: np.zeros(self.ncell) for k in ("K", "psi", "theta_l", "T", "kappa", "sink")}
+        a = {k: np.zeros(self.ncell) for k in ("K", "psi", "theta_l", "T", "kappa", "sink",
+                                               "sink_e")}
         a.update({k: np.zeros(self.ncol) for k in ("top_water", "top_heat", "bot_water",
-                                                   "bot_heat", "r_ss")})
+                                                   "bot_heat", "r_ss", "r_ess", "vapor_ice")})
         return a
 
     def _a(self, a: dict) -> Aux:
         return Aux(*(_p(a.get(k)) for k in ("K", "psi", "theta_l", "T", "kappa", "top_water",
                                             "top_heat", "bot_water", "bot_heat", "sink",
-                                            "r_ss")))
+                                            "r_ss", "sink_e", "r_ess", "vapor_ice")))
 
     @staticmethod
-    def _f(tw, th=None, bw=None, bh=None) -> Forcing:
-        return Forcing(_p(tw), _p(th), _p(bw), _p(bh))
+    def _f(tw, th=None, bw=None, bh=None, sfc=None) -> Forcing:
+        """sfc: dict of SURFACE_FIELDS arrays (AtmosDrivenFluxBC inputs) or None."""
+        s = sfc or {}
+        return Forcing(_p(tw), _p(th), _p(bw), _p(bh),
+                       *(_p(None if s.get(k) is None else
+                            np.ascontiguousarray(s[k], dtype=np.float64)) for k in SURFACE_FIELDS))
 
-    def update_cache(self, Y: dict, aux: dict, tw, th=None, bw=None, bh=None):
-        f = self._f(tw, th, bw, bh)
+    def update_cache(self, Y: dict, aux: dict, tw, th=None, bw=None, bh=None, sfc=None):
+        # contiguous float64 copies held here, so that the pointers stay valid
+        sfc = None if sfc is None else {k: np.ascontiguousarray(v, dtype=np.float64)
+                                        for k, v in sfc.items()}
+        f = self._f(tw, th, bw, bh, sfc)
         self.L.orc_update_cache(C.byref(self._m), C.byref(self._s(Y)), C.byref(f),
                                C.byref(self._a(aux)))
 
@@ -225,11 +237,13 @@ class OracleModel:
         return dx
 
     def step(self, Y: dict, dt: float, max_iters: int, tol: float, tw, th=None, bw=None,
-             bh=None, guess_v=None, guess_e=None):
+             bh=None, guess_v=None, guess_e=None, sfc=None):
         """One ARS111 step in place; guess_v / guess_e (per cell) warm-start
-        Newton at x0 = U0 + guess."""
+        Newton at x0 = U0 + guess; sfc: AtmosDrivenFluxBC inputs."""
         it = np.zeros(self.ncol, dtype=np.int32)
-        f = self._f(tw, th, bw, bh)
+        sfc = None if sfc is None else {k: np.ascontiguousarray(v, dtype=np.float64)
+                                        for k, v in sfc.items()}
+        f = self._f(tw, th, bw, bh, sfc)
         nc = self.L.orc_step_guess(C.byref(self._m), C.byref(self._s(Y)), C.byref(f), dt,
                                   max_iters, tol, it.ctypes.data_as(_ip), _p(guess_v),
                                   _p(guess_e))
