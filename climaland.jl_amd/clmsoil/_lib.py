@@ -17,12 +17,16 @@ LIB_PATH = os.environ.get("CLMSOIL_LIB", os.path.join(PKG_ROOT, "libclmsoil.so")
 CLM_OK, CLM_E_ARG, CLM_E_HIP, CLM_E_DOMAIN, CLM_E_STATE = 0, -1, -2, -3, -4
 MODEL_RICHARDS, MODEL_ENERGY_HYDROLOGY = 0, 1
 BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE, BC_TOPMODEL, BC_SURFACE_RUNOFF = 0, 1, 2, 3, 4
+BC_ATMOS, BC_ATMOS_TOPMODEL = 5, 6
 SOLVER_AUTO, SOLVER_GROUP, SOLVER_LANE = 0, 1, 2
 (P_NU, P_THETA_R, P_VG_ALPHA, P_VG_N, P_KSAT, P_SS, P_NU_SS_OM, P_NU_SS_QUARTZ,
  P_NU_SS_GRAVEL, P_KAPPA_DRY, P_KAPPA_SAT_FROZEN, P_KAPPA_SAT_UNFROZEN, P_RHOC_DS) = range(13)
 NPARAM_RRE, NPARAM_EH = 6, 13
 P_F_MAX, P_F_OVER, P_R_SB = 6, 7, 8
 NPARAM_RRE_TOPMODEL = 9
+P_EH_F_MAX, P_EH_F_OVER, P_EH_R_SB = 13, 14, 15
+NPARAM_EH_TOPMODEL = 16
+SURFACE_FIELDS = ("P_liq", "T_air", "lhf", "shf", "vapor_flux_liq", "vapor_flux_ice", "R_n")
 
 EXPORTED = (
     "clm_abi_version", "clm_create", "clm_destroy", "clm_last_error", "clm_set_state",
@@ -33,6 +37,7 @@ EXPORTED = (
     "clm_ws_init", "clm_ws_dims", "clm_ws_get_params", "clm_ws_set_params", "clm_ws_enable",
     "clm_ws_collect", "clm_ws_train", "clm_ws_features", "clm_ws_forward", "clm_ws_gradient",
     "clm_ws_time", "clm_ws_guess", "clm_save_checkpoint", "clm_read_checkpoint",
+    "clm_set_surface_fluxes", "clm_set_surface_flux_table",
 )
 
 _dp = C.POINTER(C.c_double)
@@ -57,7 +62,12 @@ class State(C.Structure):
 class Aux(C.Structure):
     _fields_ = [("K", _dp), ("psi", _dp), ("theta_l", _dp), ("T", _dp), ("kappa", _dp),
                 ("top_water", _dp), ("top_heat", _dp), ("bottom_water", _dp),
-                ("bottom_heat", _dp), ("sink", _dp), ("r_ss", _dp)]
+                ("bottom_heat", _dp), ("sink", _dp), ("r_ss", _dp), ("sink_energy", _dp),
+                ("r_ess", _dp)]
+
+
+class SurfaceFluxes(C.Structure):
+    _fields_ = [(k, _dp) for k in SURFACE_FIELDS]
 
 
 class StepStats(C.Structure):
@@ -125,6 +135,8 @@ def load(path: str | None = None) -> C.CDLL:
         "clm_ws_gradient": ([vp, _fp, _fp, _fp, C.c_int64, _fp, _fp], C.c_int),
         "clm_ws_time": ([vp, C.c_double, C.c_int32, C.POINTER(C.c_double)], C.c_int),
         "clm_ws_guess": ([vp, C.c_double, C.c_int32, _dp, _dp], C.c_int),
+        "clm_set_surface_fluxes": ([vp, C.POINTER(SurfaceFluxes)], C.c_int),
+        "clm_set_surface_flux_table": ([vp, C.c_int32, C.POINTER(SurfaceFluxes)], C.c_int),
         "clm_save_checkpoint": ([vp, C.c_char_p, C.c_double, C.c_char_p, C.c_int32], C.c_int),
         "clm_read_checkpoint": ([vp, C.c_char_p, C.POINTER(C.c_double), _ip], C.c_int),
     }

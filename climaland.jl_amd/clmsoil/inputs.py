@@ -46,6 +46,7 @@ KAPPA_QUARTZ, KAPPA_MINERALS, KAPPA_OM = 8.0, 2.5, 0.25
 KAPPA_ICE, KAPPA_LIQ = 2.21, 0.57
 RHO_L, RHO_I, CP_L, CP_I, T_0 = 1000.0, 916.7, 4181.0, 2100.0, 273.16
 LH_F0 = 2834400.0 - 2500800.0
+LH_V0, LH_S0 = 2500800.0, 2834400.0
 
 
 class ColumnRNG:
@@ -132,6 +133,7 @@ class ColumnBatch:
     top_heat: np.ndarray | None
     bc_top: int = L.BC_FLUX
     bottom_water: np.ndarray | None = None  # [ncol]: bottom flux or ϑ_bc
+    surface: dict | None = None  # AtmosDrivenFluxBC inputs {field: [nslots, ncol]}
 
     @property
     def ncol(self) -> int:
@@ -179,7 +181,9 @@ class ColumnBatch:
             self.intF_energy[cols].copy() if eh else None,
             np.ascontiguousarray(self.top_water[:, cols]),
             np.ascontiguousarray(self.top_heat[:, cols]) if eh else None, self.bc_top,
-            None if self.bottom_water is None else self.bottom_water[cols].copy())
+            None if self.bottom_water is None else self.bottom_water[cols].copy(),
+            None if self.surface is None else
+            {k: np.ascontiguousarray(v[:, cols]) for k, v in self.surface.items()})
 
     def with_moisture_state(self, top_frac: float, bottom_frac: float | None = None,
                             seed: int = 0) -> "ColumnBatch":
@@ -242,6 +246,56 @@ class ColumnBatch:
             params = P
         return dataclasses.replace(self, params=params, vartheta_l=vl,
                                    top_water=np.ascontiguousarray(precip), bc_top=kind)
+
+
+    def with_atmos(self, topmodel: bool, seed: int = 0) -> "ColumnBatch":
+        """The same EnergyHydrology columns under AtmosDrivenFluxBC (soil-only
+        form) with synthetic driver and turbulent-flux values per column and
+        slot: P_liq = -K_sat U[0.1, 3], T_air U[255, 300] K, lhf U[0, 150],
+        shf U[-50, 100], R_n U[-400, 100] W/m2, vapor_flux_liq = lhf /
+        (ρ_l L_v) on columns warmer than freezing, vapor_flux_ice likewise with
+        L_s on the others.  With TOPMODEL the columns also get the water table
+        of `with_runoff` and the TOPMODEL rows (f_max U[0.2, 0.6], f_over 3.28,
+        R_sb 1.484e-7 U[0.5, 2])."""
+        assert self.model == L.MODEL_ENERGY_HYDROLOGY
+        rng = np.random.default_rng(seed)
+        P = self.params
+        shp = self.top_water.shape
+        lhf = rng.uniform(0.0, 150.0, shp)
+        T_air = rng.uniform(255.0, 300.0, shp)
+        warm = T_air > 273.15
+        sfc = {"P_liq": -P[L.P_KSAT][None, :] * rng.uniform(0.1, 3.0, shp), "T_air": T_air,
+               "lhf": lhf, "shf": rng.uniform(-50.0, 100.0, shp),
+               "vapor_flux_liq": np.where(warm, lhf / (RHO_L * LH_V0), 0.0),
+               "vapor_flux_ice": np.where(warm, 0.0, lhf / (RHO_L * LH_S0)),
+               "R_n": rng.uniform(-400.0, 100.0, shp)}
+        b = dataclasses.replace(self, surface=sfc,
+                                bc_top=L.BC_ATMOS_TOPMODEL if topmodel else L.BC_ATMOS)
+        if not topmodel:
+            return b
+        nc = self.ncol
+        f = rng.uniform(0.1, 0.6, nc)
+        f[rng.random(nc) < 0.2] = 0.0
+        ccol = np.repeat(np.arange(nc), self.nlayers)
+        fb = np.arange(self.ncell) + ccol
+        zb = self.z_face[self.col_ptr[:-1] + np.arange(nc)]
+        zt = self.z_face[self.col_ptr[1:] + np.arange(nc)]
+        zwt = zb + f * (zt - zb)
+        zc = 0.5 * (self.z_face[fb] + self.z_face[fb + 1])
+        nu, ss = P[L.P_NU][ccol], P[L.P_SS][ccol]
+        vl = np.where(zc < zwt[ccol], nu + ss * (zwt[ccol] - zc),
+                      np.minimum(self.vartheta_l, 0.98 * nu))
+        # ρe_int keeps the column's temperature with the new liquid content
+        thl = np.minimum(self.vartheta_l, nu - self.theta_i)
+        rc_old = P[L.P_RHOC_DS][ccol] + thl * RHO_L * CP_L + self.theta_i * RHO_I * CP_I
+        T = T_0 + (self.rho_e_int + self.theta_i * RHO_I * LH_F0) / rc_old
+        thl_new = np.minimum(vl, nu - self.theta_i)
+        rc = P[L.P_RHOC_DS][ccol] + thl_new * RHO_L * CP_L + self.theta_i * RHO_I * CP_I
+        re = rc * (T - T_0) - self.theta_i * RHO_I * LH_F0
+        extra = np.stack([rng.uniform(0.2, 0.6, nc), np.full(nc, 3.28),
+                          1.484e-7 * rng.uniform(0.5, 2.0, nc)])
+        return dataclasses.replace(b, params=np.ascontiguousarray(np.concatenate([P[:13], extra])),
+                                   vartheta_l=vl, rho_e_int=re)
 
 
 def kappa_solid(nu_om, nu_q):

@@ -143,6 +143,33 @@ class TOPMODELRunoff:
 
 
 @dataclasses.dataclass
+class SoilSurfaceFluxes:
+    """What AtmosDrivenFluxBC's drivers give the soil-only boundary per column
+    (boundary_conditions.jl:880-915): p.drivers.P_liq (m/s, negative down),
+    p.drivers.T (K), and the outputs of turbulent_fluxes! (lhf, shf in W/m2;
+    vapor_flux_liq, vapor_flux_ice in m/s) and net_radiation! (R_n, W/m2).
+    SurfaceFluxes.jl's Monin-Obukhov solve that produces them stays with the
+    caller.  Each field is a scalar or one value per column."""
+    P_liq: float | np.ndarray
+    T_air: float | np.ndarray
+    lhf: float | np.ndarray = 0.0
+    shf: float | np.ndarray = 0.0
+    vapor_flux_liq: float | np.ndarray = 0.0
+    vapor_flux_ice: float | np.ndarray = 0.0
+    R_n: float | np.ndarray = 0.0
+
+
+@dataclasses.dataclass
+class AtmosDrivenFluxBC:
+    """EnergyHydrology's top boundary driven by the atmosphere
+    (boundary_conditions.jl:646-1003) in soil-only form, with NoRunoff or
+    TOPMODELRunoff; adds the SoilSublimation source (energy_hydrology.jl:909-
+    945).  `surface` holds the driver and turbulent-flux values at t_n."""
+    surface: SoilSurfaceFluxes
+    runoff: object = dataclasses.field(default_factory=NoRunoff)
+
+
+@dataclasses.dataclass
 class RichardsAtmosDrivenFluxBC:
     """boundary_conditions.jl:79-212: the top flux of a RichardsModel driven
     by the liquid precipitation rate `precip` (per column, m/s, negative
@@ -235,6 +262,8 @@ class _SoilModel:
 
     def _top_kind(self):
         top = self.boundary_conditions["top"]
+        if isinstance(top, AtmosDrivenFluxBC):
+            return L.BC_ATMOS_TOPMODEL if isinstance(top.runoff, TOPMODELRunoff) else L.BC_ATMOS
         if isinstance(top, RichardsAtmosDrivenFluxBC):
             if isinstance(top.runoff, TOPMODELRunoff):
                 return L.BC_TOPMODEL
@@ -243,8 +272,25 @@ class _SoilModel:
             return L.BC_FLUX  # NoRunoff: the precipitation is the flux
         return L.BC_MOISTURE_STATE if isinstance(top, MoistureStateBC) else L.BC_FLUX
 
+    def _surface_struct(self, sfc: SoilSurfaceFluxes, shape):
+        arrs = [np.ascontiguousarray(np.broadcast_to(getattr(sfc, k), shape), dtype=np.float64)
+                for k in L.SURFACE_FIELDS]
+        return L.SurfaceFluxes(*(L.dptr(a) for a in arrs)), arrs
+
     def set_forcing(self):
         """Push the current bc(p, t) values of the boundary conditions."""
+        top = self.boundary_conditions["top"]
+        if isinstance(top, AtmosDrivenFluxBC):
+            st, keep = self._surface_struct(top.surface, (self.domain.ncol,))
+            self._check(self._lib.clm_set_surface_fluxes(self._ctx, C.byref(st)))
+            bw = bh = None
+            b = self.boundary_conditions["bottom"]
+            if isinstance(b, WaterHeatBC):
+                bw = np.ascontiguousarray(np.broadcast_to(b.water.bc, (self.domain.ncol,)), float)
+                bh = np.ascontiguousarray(np.broadcast_to(b.heat.bc, (self.domain.ncol,)), float)
+            self._check(self._lib.clm_set_forcing(self._ctx, None, None, L.dptr(bw),
+                                                  L.dptr(bh)))
+            return
         tw, th = self._top_arrays()
         n = self.domain.ncol
         tw = np.ascontiguousarray(np.broadcast_to(tw, (n,)), dtype=np.float64)
@@ -268,11 +314,13 @@ class _SoilModel:
         if self.is_eh:
             a.update(theta_l=np.zeros(d.ncell), T=np.zeros(d.ncell), kappa=np.zeros(d.ncell),
                      top_heat=np.zeros(d.ncol), bottom_heat=np.zeros(d.ncol))
-        if self._top_kind() == L.BC_TOPMODEL:
+        if self._top_kind() in (L.BC_TOPMODEL, L.BC_ATMOS_TOPMODEL):
             a.update(sink=np.zeros(d.ncell), r_ss=np.zeros(d.ncol))
+        if self._top_kind() == L.BC_ATMOS_TOPMODEL:
+            a.update(sink_energy=np.zeros(d.ncell), r_ess=np.zeros(d.ncol))
         s = L.Aux(*(L.dptr(a.get(k)) for k in ("K", "psi", "theta_l", "T", "kappa", "top_water",
                                                "top_heat", "bottom_water", "bottom_heat",
-                                               "sink", "r_ss")))
+                                               "sink", "r_ss", "sink_energy", "r_ess")))
         self._check(self._lib.clm_get_aux(self._ctx, C.byref(s)))
         return a
 
@@ -324,6 +372,9 @@ class EnergyHydrology(_SoilModel):
         rows = [p.ν, p.θ_r, p.hydrology_cm.α, p.hydrology_cm.n, p.K_sat, p.S_s, p.ν_ss_om,
                 p.ν_ss_quartz, p.ν_ss_gravel, p.κ_dry, p.κ_sat_frozen, p.κ_sat_unfrozen,
                 p.ρc_ds]
+        top = self.boundary_conditions["top"]
+        if isinstance(top, AtmosDrivenFluxBC) and isinstance(top.runoff, TOPMODELRunoff):
+            rows += [top.runoff.f_max, top.runoff.f_over, top.runoff.R_sb]
         return np.stack([np.broadcast_to(np.asarray(r, float), (n,)) for r in rows])
 
 
@@ -548,11 +599,23 @@ class LandSimulation:
         self._on_device = False
         self.t = float(t0)
 
-    def solve(self, top_water, top_heat, nsteps: int, steps_per_slot: int,
-              want_iters: bool = False, upload: bool = True):
-        """nsteps steps with a [slot][col] forcing table, as one graph replay."""
+    def set_surface_flux_table(self, table: dict):
+        """AtmosDrivenFluxBC: {field: [slot][col] array} of SoilSurfaceFluxes fields."""
         m = self.model
-        if upload:
+        nslots = next(np.asarray(v).shape[0] for v in table.values())
+        arrs = [np.ascontiguousarray(np.broadcast_to(table.get(k, 0.0), (nslots, m.domain.ncol)),
+                                     dtype=np.float64) for k in L.SURFACE_FIELDS]
+        st = L.SurfaceFluxes(*(L.dptr(a) for a in arrs))
+        m._check(m._lib.clm_set_surface_flux_table(m._ctx, nslots, C.byref(st)))
+
+    def solve(self, top_water, top_heat, nsteps: int, steps_per_slot: int,
+              want_iters: bool = False, upload: bool = True, surface: dict | None = None):
+        """nsteps steps with a [slot][col] forcing table (or, under
+        AtmosDrivenFluxBC, the surface-flux table `surface`), as one graph replay."""
+        m = self.model
+        if upload and surface is not None:
+            self.set_surface_flux_table(surface)
+        elif upload:
             self.set_forcing_table(top_water, top_heat)
         self._push()
         st = L.StepStats()
@@ -644,9 +707,15 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
                                     P[L.P_NU_SS_OM], P[L.P_NU_SS_QUARTZ], P[L.P_NU_SS_GRAVEL],
                                     P[L.P_RHOC_DS], P[L.P_KAPPA_DRY], P[L.P_KAPPA_SAT_FROZEN],
                                     P[L.P_KAPPA_SAT_UNFROZEN])
-    bcs = {"top": WaterHeatBC(WaterFluxBC(batch.top_water[slot]),
-                              HeatFluxBC(batch.top_heat[slot])),
-           "bottom": EnergyWaterFreeDrainage()}
+    if batch.bc_top in (L.BC_ATMOS, L.BC_ATMOS_TOPMODEL):
+        runoff = TOPMODELRunoff(f_over=P[L.P_EH_F_OVER], f_max=P[L.P_EH_F_MAX],
+                                R_sb=P[L.P_EH_R_SB]) if batch.bc_top == L.BC_ATMOS_TOPMODEL \
+            else NoRunoff()
+        top = AtmosDrivenFluxBC(SoilSurfaceFluxes(**{k: v[slot] for k, v in
+                                                     batch.surface.items()}), runoff)
+    else:
+        top = WaterHeatBC(WaterFluxBC(batch.top_water[slot]), HeatFluxBC(batch.top_heat[slot]))
+    bcs = {"top": top, "bottom": EnergyWaterFreeDrainage()}
     return EnergyHydrology(parameters=par, domain=dom, boundary_conditions=bcs, **kw)
 
 
@@ -654,7 +723,7 @@ __all__ = [
     "vanGenuchten", "RichardsParameters", "EnergyHydrologyParameters", "ColumnBatchDomain",
     "WaterFluxBC", "HeatFluxBC", "FreeDrainage", "EnergyWaterFreeDrainage", "WaterHeatBC",
     "MoistureStateBC", "NewtonWarmStart", "RichardsAtmosDrivenFluxBC", "NoRunoff",
-    "SurfaceRunoff", "TOPMODELRunoff",
+    "SurfaceRunoff", "TOPMODELRunoff", "AtmosDrivenFluxBC", "SoilSurfaceFluxes",
     "RichardsModel", "EnergyHydrology", "make_update_cache", "make_update_implicit_cache",
     "make_exp_tendency", "make_compute_imp_tendency", "make_compute_jacobian",
     "initialize_jacobian", "LandSimulation", "model_from_batch",

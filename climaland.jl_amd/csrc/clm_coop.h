@@ -431,7 +431,12 @@ __global__ __launch_bounds__(256) void k_rre_coop(
 }
 
 // ---- EnergyHydrology ----------------------------------------------------------
-template <int G, int R, typename JT>
+// ATM: top boundary AtmosDrivenFluxBC in soil-only form (bc_top BC_ATMOS or
+// BC_ATMOS_TOPMODEL, surface inputs sf per column; ftw / fth are not read).
+// In update_cache!(u_n): the infiltration (TOPMODEL or the precipitation),
+// the top fluxes, and the subsurface sinks held through the step; in the
+// explicit stage the sublimation source on θ_i and ∫F_w.
+template <int G, int R, typename JT, bool ATM = false>
 __global__ __launch_bounds__(256) void k_eh_coop(
     const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
@@ -440,7 +445,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     const double* __restrict__ fth, const double* __restrict__ fbw,
     const double* __restrict__ fbh, int bc_bottom, double dt, int max_iters, double tol,
     int32_t* __restrict__ iters, const double* __restrict__ gv, const double* __restrict__ ge,
-    double* __restrict__ u0out) {
+    double* __restrict__ u0out, int bc_top, Surface sf) {
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
     const int gl = id.gl;
@@ -449,7 +454,16 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     const int N = (int)(col_ptr[col + 1] - base);
     const int64_t fbase = base + col;
     const EHPar e = load_eh(par, ncol, col);
-    const double topw = ftw[col], toph = fth[col];
+    double topw = ATM ? 0.0 : ftw[col], toph = ATM ? 0.0 : fth[col];
+    // AtmosDrivenFluxBC: subsurface runoff sinks per row, R_ss, R_ess, and the
+    // explicit ∫F_w term of sublimation
+    double snk_v[ATM ? R : 1], snk_e[ATM ? R : 1];
+    double rss = 0.0, ress = 0.0, vice = 0.0, dzt = 1.0;
+    if constexpr (ATM) {
+        vice = sf.vice[col];
+        const double ztop = zf[fbase + N];
+        dzt = ztop - 0.5 * (zf[fbase + N - 1] + ztop);
+    }
 
     double xv[R], xi[R], xe[R], u0v[R], u0e[R], zc[R], dzc[R], dzf[R];
     double Kl[R], kl[R], rcs[R], Tp[R], Pp[R], iD[R];
@@ -479,7 +493,20 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         }
         u0v[k] = v + dt * (-F);
         if (u0out && on[k]) u0out[base + r] = u0v[k];
-        xi[k] = t + dt * ((RHO_L / RHO_I) * F);
+        double dti = (RHO_L / RHO_I) * F;
+        if constexpr (ATM) {
+            if (on[k]) dti += sublimation(vice, zc[k], dzt);
+            // TOPMODEL weights of u_n: surface (with ice) in snk_e, liquid in snk_v
+            snk_e[k] = on[k] ? surface_weight(v, t, e.nu, e.v.theta_r) : 0.0;
+            snk_v[k] = on[k] ? sat_weight(v, e.nu, e.v.theta_r) : 0.0;
+            // the top layer's θ_l, θ_i, T for the infiltration capacity
+            if (on[k] && r == N - 1) {
+                rss = a.thl;
+                ress = t;
+                dzf[k] = a.T;  // scratch until the spacings are formed below
+            }
+        }
+        xi[k] = t + dt * dti;
         u0e[k] = re;
         xv[k] = u0v[k];
         xe[k] = re;
@@ -490,6 +517,47 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         Pp[k] = a.psi;
         sp[k] = make_span(e.v, e.nu - xi[k]);
         iD[k] = psi_invd(e.v, sp[k], xv[k]).invD;  // Wfact at x = U0
+    }
+    if constexpr (ATM) {
+        // update_infiltration_water_flux! (Runoff.jl:234-290) and the top fluxes
+        double Ttop = 0.0;
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (on[k] && gl * R + k == N - 1) Ttop = dzf[k];
+        const int tl = (N - 1) / R;
+        const double thl_t = g_bcast<G>(rss, tl), ti_t = g_bcast<G>(ress, tl);
+        Ttop = g_bcast<G>(Ttop, tl);
+        double infil = sf.P[col];
+        rss = ress = 0.0;
+        if (bc_top == BC_ATMOS_TOPMODEL) {
+            double hs = 0.0, h = 0.0, he = 0.0;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                hs += snk_e[k] * dzc[k];
+                h += snk_v[k] * dzc[k];
+                he += snk_v[k] * energy_liq(Tp[k]) * dzc[k];
+            }
+            hs = g_sum<G>(hs);
+            h = g_sum<G>(h);
+            he = g_sum<G>(he);
+            const double depth = zf[fbase + N] - zf[fbase];
+            const double f_over = par[P_EH_F_OVER * ncol + col];
+            double ic = eh_infil_capacity(e.v.Ksat, e.v.theta_r, thl_t, ti_t, Ttop);
+            double fsat = fmin(par[P_EH_F_MAX * ncol + col] * exp(-f_over / 2.0 * (depth - hs)),
+                               1.0);
+            infil = (1.0 - fsat) * fmax(ic, sf.P[col]);
+            rss = par[P_EH_R_SB * ncol + col] * exp(-f_over * (depth - h));
+            ress = he * (rss / fmax(h, DEPS));
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                snk_e[k] = (ress / fmax(h, DEPS)) * snk_v[k];
+                snk_v[k] = (rss / fmax(h, DEPS)) * snk_v[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < R; ++k) snk_v[k] = snk_e[k] = 0.0;
+        }
+        eh_atmos_fluxes(sf, col, infil, &topw, &toph);
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) dzf[k] = zc[k] - below<G, R>(zc, k);
@@ -621,6 +689,10 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             }
             double rv = dzc[k] * (u0v[k] - xv[k]) - dt * (Fu - F[k]);
             dRe[k] = dzc[k] * (u0e[k] - xe[k]) - dt * (Gu - Gf[k]);
+            if constexpr (ATM) {  // implicit TOPMODELSubsurfaceRunoff (Runoff.jl:346-359)
+                rv -= dt * (dzc[k] * snk_v[k]);
+                dRe[k] -= dt * (dzc[k] * snk_e[k]);
+            }
             a[k] = (JT)(dt * wK[k]);
             c[k] = (JT)(dt * wu);
             b[k] = (JT)(-dt * (wK[k] + wu) - dzc[k] * iD[k]);
@@ -703,8 +775,10 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     }
     int bad_g = (int)g_max<G>((double)bad);
     if (gl == 0) {
-        iw[col] = iw[col] + dt * (-(topw - botw));
-        ie[col] = ie[col] + dt * (-(toph - both));
+        // ∫F: U0 (explicit: sublimation) then the -I block with the step's fluxes
+        double iw0 = ATM ? iw[col] + dt * (0.0 + -vice) : iw[col];
+        iw[col] = iw0 + dt * (-(topw - botw) - rss);
+        ie[col] = ie[col] + dt * (-(toph - both) - ress);
         int32_t flag = kit_total;
         if (tol > 0.0 && !conv) flag |= 0x10000;
         if (!isfinite(err)) flag |= 0x20000;

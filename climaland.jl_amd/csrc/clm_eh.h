@@ -54,15 +54,53 @@ __device__ __forceinline__ double eh_imp_T(const EHPar& e, double vl, double ti,
     return temperature(re, ti, heat_capacity(fmin(e.nu - ti, vl), ti, e.rhoc_ds));
 }
 
+// ---- AtmosDrivenFluxBC, soil-only form (boundary_conditions.jl:880-915) --------
+// The turbulent fluxes and net radiation come from the caller (SurfaceFluxes /
+// radiation drivers are outside this path), per column:
+struct Surface {
+    const double *P, *Tair, *lhf, *shf, *vliq, *vice, *Rn;
+};
+// TOPMODEL for EnergyHydrology (Runoff.jl:234-290, 373-416): the surface pass
+// weights layers by total water (ice included) and takes the capacity
+// -K_sat F(θ_i/(θ_l+θ_i-θ_r)) G(T) of the top layer; the subsurface pass uses
+// liquid water only, and R_ess = R_ss/max(h∇,eps) Σ w ρe_l(T) Δz.
+__device__ __forceinline__ double surface_weight(double vl, double ti, double nu, double thr) {
+    double twc = vl + ti - thr, nsat = nu - thr;
+    return (twc - nsat > DEPS) ? twc / nsat : 0.0;
+}
+__device__ __forceinline__ double eh_infil_capacity(double Ksat, double thr, double thl_top,
+                                                    double ti_top, double T_top) {
+    return -Ksat * pow(10.0, -OMEGA * (ti_top / (thl_top + ti_top - thr))) *
+           exp(GAMMA_VISC * (T_top - GAMMA_T_REF));
+}
+// top fluxes from the infiltration: (water, heat)
+__device__ __forceinline__ void eh_atmos_fluxes(const Surface& s, int64_t i, double infil,
+                                                double* tw, double* th) {
+    *tw = infil + s.vliq[i];
+    *th = s.Rn[i] + s.lhf[i] + s.shf[i] + infil * energy_liq(s.Tair[i]);
+}
+// SoilSublimation source! (energy_hydrology.jl:931-945): θ_i tendency of a
+// layer centred at zc (H(z + 2 Δz_top): the top layer when the surface is at 0)
+__device__ __forceinline__ double sublimation(double vice, double zc, double dzt) {
+    return -vice * RHO_L / RHO_I * (zc + 2.0 * dzt > DEPS ? 1.0 : 0.0) / (2.0 * dzt);
+}
+
 // ---- fine-grained hooks -----------------------------------------------------
 
+// Under AtmosDrivenFluxBC (bc_top = BC_ATMOS*) sf holds the per-slot surface
+// inputs; the TOPMODEL sinks (per cell) and R_ss, R_ess (per slot) go to
+// sink_v, sink_e, rss, ress, and the ice vapour flux to vice (the explicit
+// sublimation reads it)
 __global__ __launch_bounds__(256) void k_eh_update_cache(
-    Layout L, const double* __restrict__ par, const double* __restrict__ vl,
-    const double* __restrict__ ti, const double* __restrict__ re, double* __restrict__ thl,
-    double* __restrict__ kap, double* __restrict__ T, double* __restrict__ K,
-    double* __restrict__ psi, const double* __restrict__ ftw, const double* __restrict__ fth,
-    const double* __restrict__ fbw, const double* __restrict__ fbh, double* __restrict__ tw,
-    double* __restrict__ th, double* __restrict__ bw, double* __restrict__ bh, int bc_bottom) {
+    Layout L, const double* __restrict__ par, const double* __restrict__ zf,
+    const double* __restrict__ vl, const double* __restrict__ ti, const double* __restrict__ re,
+    double* __restrict__ thl, double* __restrict__ kap, double* __restrict__ T,
+    double* __restrict__ K, double* __restrict__ psi, const double* __restrict__ ftw,
+    const double* __restrict__ fth, const double* __restrict__ fbw,
+    const double* __restrict__ fbh, double* __restrict__ tw, double* __restrict__ th,
+    double* __restrict__ bw, double* __restrict__ bh, int bc_bottom, int bc_top, Surface sf,
+    double* __restrict__ sink_v, double* __restrict__ sink_e, double* __restrict__ rss,
+    double* __restrict__ ress, double* __restrict__ vice) {
     ColView cv;
     if (!lane_column(L, cv)) return;
     EHPar e = load_eh(par, L.nslot, cv.s);
@@ -80,8 +118,46 @@ __global__ __launch_bounds__(256) void k_eh_update_cache(
             T0 = a.T;
         }
     }
-    tw[cv.s] = ftw[cv.s];
-    th[cv.s] = fth[cv.s];
+    if (bc_top == BC_ATMOS || bc_top == BC_ATMOS_TOPMODEL) {
+        const int N = cv.N;
+        const int64_t s = cv.s, ns = L.nslot;
+        double infil = sf.P[s];
+        if (bc_top == BC_ATMOS_TOPMODEL) {
+            const double thr = e.v.theta_r, nu = e.nu;
+            double hs = 0.0, h = 0.0, he = 0.0;
+            for (int j = 0; j < N; ++j) {
+                int64_t c = cv.cb + (int64_t)kWave * j;
+                double dz = zf[cv.fb + (int64_t)kWave * (j + 1)] - zf[cv.fb + (int64_t)kWave * j];
+                hs += surface_weight(vl[c], ti[c], nu, thr) * dz;
+                double w = sat_weight(vl[c], nu, thr);
+                sink_v[c] = w;
+                h += w * dz;
+                he += w * energy_liq(T[c]) * dz;
+            }
+            const int64_t ct = cv.cb + (int64_t)kWave * (N - 1);
+            double ic = eh_infil_capacity(e.v.Ksat, thr, thl[ct], ti[ct], T[ct]);
+            double depth = zf[cv.fb + (int64_t)kWave * N] - zf[cv.fb];
+            double fsat = fmin(par[P_EH_F_MAX * ns + s] *
+                                   exp(-par[P_EH_F_OVER * ns + s] / 2.0 * (depth - hs)),
+                               1.0);
+            infil = (1.0 - fsat) * fmax(ic, sf.P[s]);
+            double r = par[P_EH_R_SB * ns + s] * exp(-par[P_EH_F_OVER * ns + s] * (depth - h));
+            double re_ = he * (r / fmax(h, DEPS));
+            rss[s] = r;
+            ress[s] = re_;
+            for (int j = 0; j < N; ++j) {
+                int64_t c = cv.cb + (int64_t)kWave * j;
+                double w = sink_v[c];
+                sink_e[c] = (re_ / fmax(h, DEPS)) * w;
+                sink_v[c] = (r / fmax(h, DEPS)) * w;
+            }
+        }
+        eh_atmos_fluxes(sf, s, infil, &tw[s], &th[s]);
+        vice[s] = sf.vice[s];
+    } else {
+        tw[cv.s] = ftw[cv.s];
+        th[cv.s] = fth[cv.s];
+    }
     if (bc_bottom == BC_FREE_DRAINAGE) {  // EnergyWaterFreeDrainage, boundary_conditions.jl:590-608
         bw[cv.s] = -1.0 * K0;
         bh[cv.s] = -1.0 * K0 * energy_liq(T0);
@@ -115,11 +191,16 @@ __global__ __launch_bounds__(256) void k_eh_exp_tendency(
     const double* __restrict__ ti, const double* __restrict__ thl,
     const double* __restrict__ T, const double* __restrict__ kap, double* __restrict__ dvl,
     double* __restrict__ dti, double* __restrict__ dre, double* __restrict__ diw,
-    double* __restrict__ die, int32_t* __restrict__ bad_out) {
+    double* __restrict__ die, int32_t* __restrict__ bad_out, const double* __restrict__ vice) {
+    // vice: AtmosDrivenFluxBC's ice vapour flux per slot (sublimation), or null
     ColView cv;
     if (!lane_column(L, cv)) return;
     EHPar e = load_eh(par, L.nslot, cv.s);
     int bad = 0;
+    const int N = cv.N;
+    const double ztop = zf[cv.fb + (int64_t)kWave * N];
+    const double dzt = ztop - 0.5 * (zf[cv.fb + (int64_t)kWave * (N - 1)] + ztop);
+    const double vi = vice ? vice[cv.s] : 0.0;
     for (int j = 0; j < cv.N; ++j) {
         int64_t c = cv.cb + (int64_t)kWave * j;
         double dz = zf[cv.fb + (int64_t)kWave * (j + 1)] - zf[cv.fb + (int64_t)kWave * j];
@@ -127,10 +208,15 @@ __global__ __launch_bounds__(256) void k_eh_exp_tendency(
         double tau = 3.0 * heat_capacity(l, t, e.rhoc_ds) * dz * dz / kap[c];  // thermal_time
         double F = phase_change(e.v, e.nu, l, t, T[c], tau, &bad);
         dvl[c] = -F;
-        dti[c] = (RHO_L / RHO_I) * F;
+        double d = (RHO_L / RHO_I) * F;
+        if (vice) {
+            double zc = 0.5 * (zf[cv.fb + (int64_t)kWave * j] + zf[cv.fb + (int64_t)kWave * (j + 1)]);
+            d += sublimation(vi, zc, dzt);
+        }
+        dti[c] = d;
         dre[c] = 0.0;
     }
-    diw[cv.s] = 0.0;
+    diw[cv.s] = vice ? 0.0 + -vi : 0.0;
     die[cv.s] = 0.0;
     if (bad) bad_out[cv.s] = 1;
 }
@@ -142,13 +228,16 @@ __global__ __launch_bounds__(256) void k_eh_imp_tendency(
     const double* __restrict__ kap, const double* __restrict__ tw,
     const double* __restrict__ th, const double* __restrict__ bw,
     const double* __restrict__ bh, double* __restrict__ dvl, double* __restrict__ dti,
-    double* __restrict__ dre, double* __restrict__ diw, double* __restrict__ die) {
+    double* __restrict__ dre, double* __restrict__ diw, double* __restrict__ die,
+    const double* __restrict__ sink_v, const double* __restrict__ sink_e,
+    const double* __restrict__ rss, const double* __restrict__ ress) {
+    // sink_*, rss, ress: TOPMODEL subsurface runoff of EH (Runoff.jl:346-359), or null
     ColView cv;
     if (!lane_column(L, cv)) return;
     const int N = cv.N;
     double topw = tw[cv.s], toph = th[cv.s], botw = bw[cv.s], both = bh[cv.s];
-    diw[cv.s] = -(topw - botw);
-    die[cv.s] = -(toph - both);
+    diw[cv.s] = sink_v ? -(topw - botw) - rss[cv.s] : -(topw - botw);
+    die[cv.s] = sink_v ? -(toph - both) - ress[cv.s] : -(toph - both);
     double f0 = zf[cv.fb], f1 = zf[cv.fb + kWave];
     double zc_c = 0.5 * (f0 + f1), dzc_c = f1 - f0;
     double K_c = K[cv.cb], h_c = psi[cv.cb] + zc_c, T_c = T[cv.cb], k_c = kap[cv.cb];
@@ -175,8 +264,13 @@ __global__ __launch_bounds__(256) void k_eh_imp_tendency(
             f1 = f2;
         }
         int64_t c = cv.cb + (int64_t)kWave * i;
-        dvl[c] = -(Fhi - Flo) / dzc_c;
-        dre[c] = -(Ghi - Glo) / dzc_c;
+        double tv = -(Fhi - Flo) / dzc_c, te = -(Ghi - Glo) / dzc_c;
+        if (sink_v) {
+            tv -= sink_v[c];
+            te -= sink_e[c];
+        }
+        dvl[c] = tv;
+        dre[c] = te;
         dti[c] = 0.0;
         Flo = Fhi;
         Glo = Ghi;

@@ -18,7 +18,10 @@ namespace clm {
 // RichardsAtmosDrivenFluxBC(precipitation, runoff) (boundary_conditions.jl:79-212)
 // with TOPMODELRunoff or SurfaceRunoff; the forcing value is the precipitation
 enum { BC_FLUX = 0, BC_FREE_DRAINAGE = 1, BC_MOISTURE_STATE = 2, BC_TOPMODEL = 3,
-       BC_SURFACE_RUNOFF = 4 };
+       BC_SURFACE_RUNOFF = 4,
+       // EnergyHydrology top: AtmosDrivenFluxBC in soil-only form with the
+       // surface fluxes supplied per column, NoRunoff or TOPMODELRunoff
+       BC_ATMOS = 5, BC_ATMOS_TOPMODEL = 6 };
 // RichardsModel kernels take both boundary kinds packed as bottom | top << 4
 __host__ __device__ inline int bc_bot(int bc) { return bc & 15; }
 __host__ __device__ inline int bc_top(int bc) { return bc >> 4; }
@@ -79,6 +82,8 @@ enum {
 };
 // RichardsModel under a TOPMODEL top boundary: rows 6-8
 enum { P_F_MAX = 6, P_F_OVER = 7, P_R_SB = 8, NPARAM_RRE_TOPMODEL = 9 };
+// EnergyHydrology under AtmosDrivenFluxBC + TOPMODEL: rows 13-15
+enum { P_EH_F_MAX = 13, P_EH_F_OVER = 14, P_EH_R_SB = 15, NPARAM_MAX = 16 };
 
 __device__ __forceinline__ bool lane_column(const Layout& L, ColView& cv) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -60,6 +60,14 @@ struct clm_ctx {
     double *d_K = nullptr, *d_psi = nullptr, *d_thl = nullptr, *d_T = nullptr, *d_kap = nullptr;
     double *d_tw = nullptr, *d_th = nullptr, *d_bw = nullptr, *d_bh = nullptr;
     double *d_sink = nullptr, *d_rss = nullptr;  // TOPMODEL subsurface runoff
+    // EnergyHydrology AtmosDrivenFluxBC: energy sink, R_ess, ice vapour flux
+    double *d_sinke = nullptr, *d_ress = nullptr, *d_vice = nullptr;
+    // its surface inputs (P_liq, T_air, lhf, shf, vapor_liq, vapor_ice, R_n):
+    // slot order (hooks), column order (group kernels), [slot][col] table
+    double* d_sfc[7] = {nullptr};
+    double* d_csfc[7] = {nullptr};
+    double* d_tab_sfc[7] = {nullptr};
+    int32_t sfc_slots = 0;
     bool have_cache = false, have_jac = false;
     // fine-hook work buffers
     double* d_wy[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -409,7 +417,30 @@ struct StepExtra {
     const double* gv = nullptr;
     const double* ge = nullptr;
     double* u0out = nullptr;
+    Surface sf{};  // AtmosDrivenFluxBC inputs per column (EH)
 };
+
+bool is_atmos(const clm_ctx* c) {
+    return c->model == CLM_MODEL_ENERGY_HYDROLOGY &&
+           (c->bc_top == CLM_BC_ATMOS || c->bc_top == CLM_BC_ATMOS_TOPMODEL);
+}
+Surface surface_of(double* const* a, int64_t off) {
+    Surface s;
+    s.P = a[0] + off;
+    s.Tair = a[1] + off;
+    s.lhf = a[2] + off;
+    s.shf = a[3] + off;
+    s.vliq = a[4] + off;
+    s.vice = a[5] + off;
+    s.Rn = a[6] + off;
+    return s;
+}
+// extras of a plain step on the current forcing
+StepExtra current_extra(const clm_ctx* c) {
+    StepExtra x;
+    if (is_atmos(c)) x.sf = surface_of(c->d_csfc, 0);
+    return x;
+}
 
 template <typename JT, int G, int R>
 void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const double* fth,
@@ -421,10 +452,18 @@ void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const dou
                            bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ciw, ftw, c->d_cfbw,
                            rre_bc(c), dt, c->max_iters, c->tol, iters, x.gv, x.u0out);
     } else {
-        hipLaunchKernelGGL((k_eh_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
-                           bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci, c->d_ce,
-                           c->d_ciw, c->d_cie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt,
-                           c->max_iters, c->tol, iters, x.gv, x.ge, x.u0out);
+        if (is_atmos(c))
+            hipLaunchKernelGGL((k_eh_coop<G, R, JT, true>), g, b, 0, c->stream, c->d_col_ptr,
+                               bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci,
+                               c->d_ce, c->d_ciw, c->d_cie, ftw, fth, c->d_cfbw, c->d_cfbh,
+                               c->bc_bottom, dt, c->max_iters, c->tol, iters, x.gv, x.ge,
+                               x.u0out, c->bc_top, x.sf);
+        else
+            hipLaunchKernelGGL((k_eh_coop<G, R, JT, false>), g, b, 0, c->stream, c->d_col_ptr,
+                               bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci,
+                               c->d_ce, c->d_ciw, c->d_cie, ftw, fth, c->d_cfbw, c->d_cfbh,
+                               c->bc_bottom, dt, c->max_iters, c->tol, iters, x.gv, x.ge,
+                               x.u0out, c->bc_top, x.sf);
     }
 }
 
@@ -547,9 +586,10 @@ void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt
                      StepExtra* x);
 
 void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters,
-                 double* u0out = nullptr, bool allow_warm = true) {
+                 double* u0out = nullptr, bool allow_warm = true, const Surface* sf = nullptr) {
     if (is_group(c)) {
-        StepExtra x;
+        StepExtra x = current_extra(c);
+        if (sf) x.sf = *sf;
         x.u0out = u0out;
         if (allow_warm) ws_launch_guess(c, ftw, fth, dt, &x);
         if (c->jac_prec == 32)
@@ -863,8 +903,14 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
         cfg->bc_bottom != CLM_BC_MOISTURE_STATE)
         return set_err(nullptr, CLM_E_ARG, "unknown bottom boundary %d", cfg->bc_bottom);
     if (cfg->bc_top != CLM_BC_FLUX && cfg->bc_top != CLM_BC_MOISTURE_STATE &&
-        cfg->bc_top != CLM_BC_TOPMODEL && cfg->bc_top != CLM_BC_SURFACE_RUNOFF)
+        cfg->bc_top != CLM_BC_TOPMODEL && cfg->bc_top != CLM_BC_SURFACE_RUNOFF &&
+        cfg->bc_top != CLM_BC_ATMOS && cfg->bc_top != CLM_BC_ATMOS_TOPMODEL)
         return set_err(nullptr, CLM_E_ARG, "unknown top boundary %d", cfg->bc_top);
+    const bool atmos = cfg->bc_top == CLM_BC_ATMOS || cfg->bc_top == CLM_BC_ATMOS_TOPMODEL;
+    if (atmos && cfg->model != CLM_MODEL_ENERGY_HYDROLOGY)
+        return set_err(nullptr, CLM_E_ARG,
+                       "AtmosDrivenFluxBC is the EnergyHydrology top boundary; RichardsModel "
+                       "takes RichardsAtmosDrivenFluxBC (CLM_BC_TOPMODEL / SURFACE_RUNOFF)");
     if (cfg->model == CLM_MODEL_ENERGY_HYDROLOGY &&
         (cfg->bc_top == CLM_BC_MOISTURE_STATE || cfg->bc_bottom == CLM_BC_MOISTURE_STATE))
         return set_err(nullptr, CLM_E_ARG, "MoistureStateBC is implemented for RichardsModel only");
@@ -874,7 +920,8 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
                        "RichardsAtmosDrivenFluxBC runoff is implemented for RichardsModel only");
     int need = cfg->model == CLM_MODEL_RICHARDS
                    ? (cfg->bc_top == CLM_BC_TOPMODEL ? CLM_NPARAM_RRE_TOPMODEL : CLM_NPARAM_RRE)
-                   : CLM_NPARAM_EH;
+                   : (cfg->bc_top == CLM_BC_ATMOS_TOPMODEL ? CLM_NPARAM_EH_TOPMODEL
+                                                           : CLM_NPARAM_EH);
     if (cfg->nparam < need)
         return set_err(nullptr, CLM_E_ARG, "nparam %d < %d required", cfg->nparam, need);
     if (cfg->ncol <= 0 || cfg->ncol > (int64_t)1 << 30 || !cfg->col_ptr || !cfg->z_face ||
@@ -908,6 +955,10 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     if (solver == CLM_SOLVER_AUTO) solver = nmax_all <= 128 ? CLM_SOLVER_GROUP : CLM_SOLVER_LANE;
     if (solver == CLM_SOLVER_GROUP && nmax_all > 128)
         return set_err(nullptr, CLM_E_ARG, "group solver takes columns of up to 128 layers");
+    if (atmos && solver != CLM_SOLVER_GROUP)
+        return set_err(nullptr, CLM_E_ARG,
+                       "AtmosDrivenFluxBC is implemented for the group solver (columns of up "
+                       "to 128 layers)");
 
     clm_ctx* c = new clm_ctx();
     c->solver = solver;
@@ -972,7 +1023,7 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     ALLOC(c->d_face_off, c->npack + 1);
     ALLOC(c->d_nl, c->nslot);
     ALLOC(c->d_slot_col, c->nslot);
-    ALLOC(c->d_par, (int64_t)NPARAM_EH * c->nslot);
+    ALLOC(c->d_par, (int64_t)NPARAM_MAX * c->nslot);
     ALLOC(c->d_zf, c->npface);
     ALLOC(c->d_xv, c->npcell);
     ALLOC(c->d_iw, c->nslot);
@@ -983,9 +1034,20 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     ALLOC(c->d_K, c->npcell);
     ALLOC(c->d_psi, c->npcell);
     ALLOC(c->d_tw, c->nslot);
-    if (c->bc_top == CLM_BC_TOPMODEL) {
+    if (c->bc_top == CLM_BC_TOPMODEL || c->bc_top == CLM_BC_ATMOS_TOPMODEL) {
         ALLOC(c->d_sink, c->npcell);
         ALLOC(c->d_rss, c->nslot);
+    }
+    if (c->bc_top == CLM_BC_ATMOS_TOPMODEL) {
+        ALLOC(c->d_sinke, c->npcell);
+        ALLOC(c->d_ress, c->nslot);
+    }
+    if (is_atmos(c)) {
+        ALLOC(c->d_vice, c->nslot);
+        for (int k = 0; k < 7; ++k) {
+            ALLOC(c->d_sfc[k], c->nslot);
+            ALLOC(c->d_csfc[k], c->ncol);
+        }
     }
     ALLOC(c->d_th, c->nslot);
     ALLOC(c->d_bw, c->nslot);
@@ -1016,7 +1078,7 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     }
     if (is_group(c)) {
         ALLOC(c->d_col_ptr, c->ncol + 1);
-        ALLOC(c->d_cpar, (int64_t)NPARAM_EH * c->ncol);
+        ALLOC(c->d_cpar, (int64_t)NPARAM_MAX * c->ncol);
         ALLOC(c->d_czf, c->ncell + c->ncol);
         ALLOC(c->d_cv, c->ncell);
         ALLOC(c->d_ciw, c->ncol);
@@ -1078,7 +1140,7 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     }
     // ---- upload layout, parameters, faces ----
     {
-        std::vector<double> par((size_t)NPARAM_EH * c->nslot, 0.0);
+        std::vector<double> par((size_t)NPARAM_MAX * c->nslot, 0.0);
         for (int p = 0; p < need; ++p)
             for (int64_t s = 0; s < c->nslot; ++s) {
                 int32_t col = c->slot_col[s];
@@ -1104,7 +1166,7 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
         up(c->d_par, par.data(), 8 * par.size());
         up(c->d_zf, zf.data(), 8 * zf.size());
         if (is_group(c)) {
-            std::vector<double> cpar((size_t)NPARAM_EH * c->ncol, 1.0);
+            std::vector<double> cpar((size_t)NPARAM_MAX * c->ncol, 1.0);
             for (int p = 0; p < need; ++p)
                 std::memcpy(cpar.data() + (size_t)p * c->ncol, cfg->params + (size_t)p * c->ncol,
                             8 * (size_t)c->ncol);
@@ -1157,6 +1219,14 @@ int clm_destroy(clm_ctx* c) {
     dfree(c->d_tw);
     dfree(c->d_sink);
     dfree(c->d_rss);
+    dfree(c->d_sinke);
+    dfree(c->d_ress);
+    dfree(c->d_vice);
+    for (int k = 0; k < 7; ++k) {
+        dfree(c->d_sfc[k]);
+        dfree(c->d_csfc[k]);
+        dfree(c->d_tab_sfc[k]);
+    }
     dfree(c->d_th);
     dfree(c->d_bw);
     dfree(c->d_bh);
@@ -1284,6 +1354,45 @@ int clm_set_forcing_table(clm_ctx* c, int32_t nslots, const double* tw, const do
     return CLM_OK;
 }
 
+static const double* sfc_field(const clm_surface_fluxes* s, int k) {
+    const double* f[7] = {s->P_liq, s->T_air, s->lhf, s->shf, s->vapor_flux_liq,
+                          s->vapor_flux_ice, s->R_n};
+    return f[k];
+}
+
+int clm_set_surface_fluxes(clm_ctx* c, const clm_surface_fluxes* s) {
+    if (!c || !s) return CLM_E_ARG;
+    if (!is_atmos(c)) return set_err(c, CLM_E_STATE, "the top boundary is not AtmosDrivenFluxBC");
+    int r;
+    for (int k = 0; k < 7; ++k) {
+        if ((r = upload_cols(c, sfc_field(s, k), c->d_sfc[k]))) return r;
+        if ((r = upload_raw(c, sfc_field(s, k), c->d_csfc[k], c->ncol))) return r;
+    }
+    return CLM_OK;
+}
+
+int clm_set_surface_flux_table(clm_ctx* c, int32_t nslots, const clm_surface_fluxes* s) {
+    if (!c || !s) return CLM_E_ARG;
+    if (!is_atmos(c)) return set_err(c, CLM_E_STATE, "the top boundary is not AtmosDrivenFluxBC");
+    if (nslots < 1) return set_err(c, CLM_E_ARG, "surface flux table needs nslots >= 1");
+    int r;
+    if (nslots != c->sfc_slots) {
+        for (int k = 0; k < 7; ++k) {
+            dfree(c->d_tab_sfc[k]);
+            c->d_tab_sfc[k] = nullptr;
+        }
+        c->sfc_slots = 0;
+        for (int k = 0; k < 7; ++k)
+            if ((r = dalloc(c, &c->d_tab_sfc[k], (int64_t)nslots * c->ncol))) return r;
+        c->sfc_slots = nslots;
+        destroy_graph(c);
+    }
+    for (int k = 0; k < 7; ++k)
+        if ((r = upload_raw(c, sfc_field(s, k), c->d_tab_sfc[k], (int64_t)nslots * c->ncol)))
+            return r;
+    return CLM_OK;
+}
+
 // ---- fine-grained hooks ------------------------------------------------------
 
 int clm_update_cache(clm_ctx* c, const clm_state* Y, double t) {
@@ -1297,10 +1406,14 @@ int clm_update_cache(clm_ctx* c, const clm_state* Y, double t) {
                            c->d_par, c->d_zf, c->d_wy[0], c->d_K, c->d_psi, c->d_ftw, c->d_fbw,
                            c->d_tw, c->d_bw, rre_bc(c), 1, c->d_sink, c->d_rss);
     } else {
+        Surface sf{};
+        if (is_atmos(c)) sf = surface_of(c->d_sfc, 0);
         hipLaunchKernelGGL(k_eh_update_cache, grid_slots(c), dim3(256), 0, c->stream, L,
-                           c->d_par, c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_thl, c->d_kap,
-                           c->d_T, c->d_K, c->d_psi, c->d_ftw, c->d_fth, c->d_fbw, c->d_fbh,
-                           c->d_tw, c->d_th, c->d_bw, c->d_bh, c->bc_bottom);
+                           c->d_par, c->d_zf, c->d_wy[0], c->d_wy[1], c->d_wy[2], c->d_thl,
+                           c->d_kap, c->d_T, c->d_K, c->d_psi, c->d_ftw, c->d_fth, c->d_fbw,
+                           c->d_fbh, c->d_tw, c->d_th, c->d_bw, c->d_bh, c->bc_bottom,
+                           c->bc_top, sf, c->d_sink, c->d_sinke, c->d_rss, c->d_ress,
+                           c->d_vice);
     }
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1345,7 +1458,7 @@ int clm_exp_tendency(clm_ctx* c, const clm_state* Y, double t, clm_state* dY) {
         hipLaunchKernelGGL(k_eh_exp_tendency, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_par, c->d_zf, c->d_wy[1], c->d_thl, c->d_T, c->d_kap,
                            c->d_wd[0], c->d_wd[1], c->d_wd[2], c->d_wd[3], c->d_wd[4],
-                           c->d_bad);
+                           c->d_bad, is_atmos(c) ? c->d_vice : nullptr);
         HIPCHK(c, hipGetLastError());
     }
     r = download_state(c, c->d_wd, dY);
@@ -1377,7 +1490,8 @@ int clm_imp_tendency(clm_ctx* c, const clm_state* Y, double t, clm_state* dY) {
         hipLaunchKernelGGL(k_eh_imp_tendency, grid_slots(c), dim3(256), 0, c->stream, L,
                            c->d_zf, c->d_K, c->d_psi, c->d_T, c->d_kap, c->d_tw, c->d_th,
                            c->d_bw, c->d_bh, c->d_wd[0], c->d_wd[1], c->d_wd[2], c->d_wd[3],
-                           c->d_wd[4]);
+                           c->d_wd[4], c->d_sinke ? c->d_sink : nullptr, c->d_sinke, c->d_rss,
+                           c->d_ress);
     }
     HIPCHK(c, hipGetLastError());
     return download_state(c, c->d_wd, dY);
@@ -1476,6 +1590,10 @@ int clm_get_aux(clm_ctx* c, clm_aux* p) {
         if ((r = download_cells(c, c->d_sink, p->sink))) return r;
         if ((r = download_cols(c, c->d_rss, p->r_ss))) return r;
     }
+    if (c->d_sinke) {
+        if ((r = download_cells(c, c->d_sinke, p->sink_energy))) return r;
+        if ((r = download_cols(c, c->d_ress, p->r_ess))) return r;
+    }
     if (is_eh(c)) {
         if ((r = download_cells(c, c->d_thl, p->theta_l))) return r;
         if ((r = download_cells(c, c->d_T, p->T))) return r;
@@ -1558,18 +1676,28 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
     if (!c) return CLM_E_ARG;
     if (nsteps < 1 || steps_per_slot < 1 || !(dt > 0.0))
         return set_err(c, CLM_E_ARG, "clm_run: bad nsteps/steps_per_slot/dt");
-    if (c->tab_slots < 1) return set_err(c, CLM_E_STATE, "clm_set_forcing_table must run first");
+    const bool atm = is_atmos(c);
+    const int32_t nslots = atm ? c->sfc_slots : c->tab_slots;
+    if (nslots < 1)
+        return set_err(c, CLM_E_STATE, atm ? "clm_set_surface_flux_table must run first"
+                                           : "clm_set_forcing_table must run first");
     int r = ensure_iters(c, nsteps);
     if (r) return r;
     if (!c->gexec || c->g_nsteps != nsteps || c->g_sps != steps_per_slot ||
-        c->g_tab != c->tab_slots || c->g_dt != dt) {
+        c->g_tab != nslots || c->g_dt != dt) {
         destroy_graph(c);
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int32_t s = 0; s < nsteps; ++s) {
-            int64_t slot = (s / steps_per_slot) % c->tab_slots;
+            int64_t slot = (s / steps_per_slot) % nslots;
             int64_t nv = ncolvec(c);
-            launch_step(c, c->d_tab_w + slot * nv, c->d_tab_h + slot * nv, dt,
-                        c->d_iters + (int64_t)s * nv);
+            if (atm) {
+                Surface sf = surface_of(c->d_tab_sfc, slot * c->ncol);
+                launch_step(c, nullptr, nullptr, dt, c->d_iters + (int64_t)s * nv, nullptr,
+                            false, &sf);
+            } else {
+                launch_step(c, c->d_tab_w + slot * nv, c->d_tab_h + slot * nv, dt,
+                            c->d_iters + (int64_t)s * nv);
+            }
         }
         hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
         if (e != hipSuccess) {
@@ -1579,7 +1707,7 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
         HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
         c->g_nsteps = nsteps;
         c->g_sps = steps_per_slot;
-        c->g_tab = c->tab_slots;
+        c->g_tab = nslots;
         c->g_dt = dt;
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -1737,9 +1865,11 @@ int clm_autotune(clm_ctx* c, double dt, int32_t reps) {
                 if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
                 if (e == hipSuccess) {
                     if (c->jac_prec == 32)
-                        launch_bin_any<float>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters);
+                        launch_bin_any<float>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters,
+                                              current_extra(c));
                     else
-                        launch_bin_any<double>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters);
+                        launch_bin_any<double>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters,
+                                               current_extra(c));
                     e = hipGetLastError();
                 }
                 if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
@@ -1779,6 +1909,9 @@ int clm_last_kernel_ms(const clm_ctx* c, double* ms_per_launch, int32_t* launche
 int clm_ws_init(clm_ctx* c, uint64_t seed) {
     if (!c) return CLM_E_ARG;
     if (!is_group(c)) return set_err(c, CLM_E_STATE, "the warm start needs the group solver");
+    if (is_atmos(c))
+        return set_err(c, CLM_E_STATE, "the warm start takes prescribed top fluxes, not "
+                                       "AtmosDrivenFluxBC");
     ws_free(c->ws);
     c->ws = nullptr;
     destroy_graph(c);

@@ -81,7 +81,15 @@ enum {
                                   EnergyWaterFreeDrainage (EH, :590-608) */
     CLM_BC_MOISTURE_STATE = 2, /* MoistureStateBC (RRE) */
     CLM_BC_TOPMODEL = 3,       /* top, RRE: TOPMODELRunoff (Runoff.jl:199-271) */
-    CLM_BC_SURFACE_RUNOFF = 4  /* top, RRE: SurfaceRunoff (Runoff.jl:84-148) */
+    CLM_BC_SURFACE_RUNOFF = 4, /* top, RRE: SurfaceRunoff (Runoff.jl:84-148) */
+    /* top, EH: AtmosDrivenFluxBC(atmos, radiation, runoff) in soil-only form
+     * (boundary_conditions.jl:646-1003, energy_hydrology.jl:909-945) with the
+     * turbulent fluxes and net radiation supplied per column
+     * (clm_set_surface_fluxes): NoRunoff, or TOPMODELRunoff with its
+     * subsurface sinks of water and energy; adds the SoilSublimation source.
+     * Group solver only. */
+    CLM_BC_ATMOS = 5,
+    CLM_BC_ATMOS_TOPMODEL = 6
 };
 
 /* Per-column parameter rows.  RRE uses the first CLM_NPARAM_RRE rows
@@ -109,6 +117,9 @@ enum {
  * column), f_over (1/m) and R_sb (m/s) */
 enum { CLM_P_F_MAX = 6, CLM_P_F_OVER = 7, CLM_P_R_SB = 8 };
 #define CLM_NPARAM_RRE_TOPMODEL 9
+/* EH under CLM_BC_ATMOS_TOPMODEL: the same three rows after the EH rows */
+enum { CLM_P_EH_F_MAX = 13, CLM_P_EH_F_OVER = 14, CLM_P_EH_R_SB = 15 };
+#define CLM_NPARAM_EH_TOPMODEL 16
 
 /* Coarse-path solvers (clm_step / clm_run).  Both solve the same
  * block-tridiagonal Newton systems; they differ in how a column maps onto
@@ -164,10 +175,27 @@ typedef struct {
     double* top_heat;    /* p.soil.top_bc.heat */
     double* bottom_water;
     double* bottom_heat;
-    double* sink;        /* CLM_BC_TOPMODEL: subsurface runoff sink per cell (1/s),
-                            (R_ss / max(h∇, eps)) · is_saturated (Runoff.jl:321-332) */
-    double* r_ss;        /* CLM_BC_TOPMODEL: p.soil.R_ss per column (m/s) */
+    double* sink;        /* CLM_BC_TOPMODEL / ATMOS_TOPMODEL: subsurface runoff sink per
+                            cell (1/s), (R_ss / max(h∇, eps)) · is_saturated
+                            (Runoff.jl:321-332, 346-359) */
+    double* r_ss;        /* p.soil.R_ss per column (m/s) */
+    double* sink_energy; /* CLM_BC_ATMOS_TOPMODEL: (R_ess / max(h∇, eps)) · is_saturated
+                            per cell (W/m3) */
+    double* r_ess;       /* CLM_BC_ATMOS_TOPMODEL: p.soil.R_ess per column (W/m2) */
 } clm_aux;
+
+/* AtmosDrivenFluxBC inputs per column (CLM_BC_ATMOS*): what turbulent_fluxes!,
+ * net_radiation! and the drivers give the soil-only boundary in the reference
+ * (boundary_conditions.jl:880-915). */
+typedef struct {
+    const double* P_liq;          /* p.drivers.P_liq (m/s, negative downward) */
+    const double* T_air;          /* p.drivers.T (K): energy of the infiltrating water */
+    const double* lhf;            /* p.soil.turbulent_fluxes.lhf (W/m2) */
+    const double* shf;            /* p.soil.turbulent_fluxes.shf (W/m2) */
+    const double* vapor_flux_liq; /* p.soil.turbulent_fluxes.vapor_flux_liq (m/s) */
+    const double* vapor_flux_ice; /* p.soil.turbulent_fluxes.vapor_flux_ice (m/s) */
+    const double* R_n;            /* p.soil.R_n (W/m2) */
+} clm_surface_fluxes;
 
 typedef struct {
     int32_t max_iters_used;   /* max Newton iterations over columns */
@@ -196,6 +224,11 @@ int clm_set_forcing(clm_ctx* ctx, const double* top_water, const double* top_hea
  * ([slot][col]); step s of a run uses slot (s / steps_per_slot) % nslots. */
 int clm_set_forcing_table(clm_ctx* ctx, int32_t nslots, const double* top_water,
                           const double* top_heat);
+/* CLM_BC_ATMOS*: the surface inputs at t_n for clm_step and the hooks, and
+ * an [slot][col] schedule of them for clm_run (step s uses slot
+ * (s / steps_per_slot) % nslots).  NULL members mean zero. */
+int clm_set_surface_fluxes(clm_ctx* ctx, const clm_surface_fluxes* fluxes);
+int clm_set_surface_flux_table(clm_ctx* ctx, int32_t nslots, const clm_surface_fluxes* fluxes);
 
 /* ---- fine-grained hooks (1:1 with the ClimaLand closures) ----------------
  * Y / dY are host CSR buffers.  The cache p lives on the device inside ctx;

@@ -33,7 +33,7 @@ class Model(C.Structure):
 
 BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE, BC_TOPMODEL, BC_SURFACE_RUNOFF = 0, 1, 2, 3, 4
 BC_ATMOS, BC_ATMOS_TOPMODEL = 5, 6
-SURFACE_FIELDS = ("P_liq", "T_air", "lhf", "shf", "vapor_liq", "vapor_ice", "R_n")
+SURFACE_FIELDS = ("P_liq", "T_air", "lhf", "shf", "vapor_flux_liq", "vapor_flux_ice", "R_n")
 
 
 class State(C.Structure):

@@ -51,7 +51,8 @@ def test_struct_layouts_match_header():
     from clmsoil import _lib
     assert C.sizeof(_lib.Config) == 4 + 4 + 8 + 8 * 3 + 4 * 3 + 4 + 8 + 4 + 4  # incl. padding
     assert C.sizeof(_lib.State) == 5 * 8
-    assert C.sizeof(_lib.Aux) == 11 * 8
+    assert C.sizeof(_lib.Aux) == 13 * 8
+    assert C.sizeof(_lib.SurfaceFluxes) == 7 * 8
     assert C.sizeof(_lib.StepStats) == 4 + 4 + 8 + 8 * 3 + 8
 
 
@@ -68,7 +69,12 @@ def test_struct_layouts_match_header():
     (lambda c: setattr(c, "bc_top", 3), -1),          # TOPMODEL needs 9 parameter rows
     (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 3)), -1),
     (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 4)), -1),
-    (lambda c: setattr(c, "bc_top", 5), -1),
+    (lambda c: setattr(c, "bc_top", 5), -1),          # AtmosDrivenFluxBC is EH only
+    # EH + TOPMODEL atmos needs 16 parameter rows
+    (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 6)), -1),
+    # EH + atmos on the lane solver
+    (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 5),
+                setattr(c, "solver", 2)), -1),
 ])
 def test_create_rejects_bad_config_without_device(mutate, code):
     """Validation runs before any HIP call, so this is safe without a GPU."""

@@ -54,7 +54,8 @@ def state(o, vl, ti, T):
 
 def sfc(P=-2e-5, T_air=285.0, lhf=-80.0, shf=-20.0, vliq=3e-8, vice=0.0, R_n=-300.0):
     return {k: np.array([v]) for k, v in dict(P_liq=P, T_air=T_air, lhf=lhf, shf=shf,
-                                             vapor_liq=vliq, vapor_ice=vice, R_n=R_n).items()}
+                                             vapor_flux_liq=vliq, vapor_flux_ice=vice,
+                                             R_n=R_n).items()}
 
 
 def cache(o, Y, s):
