@@ -36,6 +36,17 @@ HBM_PEAK_GBS = 8000.0
 MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1218.6e6, "raw": 709.3e6},
                     ("eh_262144_uneven", "mixed"): {"scaled": 1049.3e6, "raw": 624.7e6}}
 
+# What bounds the fused step: VALU issue.  rocprofv3 counters of the step
+# kernels on 3 early steps of eh_262144_uneven, mixed form, shapes
+# (16,1),(32,1),(32,1),(64,1) (profiles/k_eh_coop_eh_262144_uneven.md):
+# SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes per step, SQ_INSTS_VALU per step, and
+# the kernels' time per step in the same build's trace.
+MEASURED_VALU = {"eh_262144_uneven": {"fp64_flop_per_step": 7.546e10,
+                                      "valu_instructions_per_step": 1.842e9,
+                                      "kernel_ms_per_step": 4.009}}
+FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
+CU_COUNT, ENGINE_GHZ = 256, 2.4
+
 
 def step_bytes(batch, learned: bool) -> float:
     """Compulsory HBM bytes of one fused step over all columns (the group
@@ -315,6 +326,17 @@ def main():
                      "unfused_equivalent_frac": unfused / (r["kernel_ms"] * 1e-3) / 1e9
                      / HBM_PEAK_GBS},
     }
+    v = MEASURED_VALU.get(w.name)
+    if v is not None:
+        t = v["kernel_ms_per_step"] * 1e-3
+        out["valu_roofline"] = {
+            "kernel": kname, "source": "rocprofv3 counters, 3 early steps, mixed form",
+            "fp64_tflops": v["fp64_flop_per_step"] / t / 1e12,
+            "fp64_peak_tflops": FP64_VALU_PEAK_TFLOPS,
+            "fp64_frac": v["fp64_flop_per_step"] / t / 1e12 / FP64_VALU_PEAK_TFLOPS,
+            "valu_issue_per_cu_cycle": v["valu_instructions_per_step"] /
+            (CU_COUNT * t * ENGINE_GHZ * 1e9),
+            "valu_issue_peak_per_cu_cycle": 1.0}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         sample = args.cpu_sample or min(batch.ncol, 24576 if batch.model == 1 else 16384)
