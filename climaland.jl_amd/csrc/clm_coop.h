@@ -51,13 +51,26 @@ __device__ __forceinline__ double g_max(double v) {
     return v;
 }
 
+// Reciprocal of a pivot: exact division in FP64; in the FP32 solve of the
+// mixed form (an inexact Newton whose FP64 residual decides convergence) the
+// hardware reciprocal (1 ulp), one instruction instead of the ten of an IEEE
+// division.  The pivots are those of a strictly diagonally dominant system:
+// never zero, never denormal.
+template <typename JT>
+__device__ __forceinline__ JT recip(JT x) {
+    if constexpr (std::is_same<JT, float>::value)
+        return __builtin_amdgcn_rcpf(x);
+    else
+        return (JT)1 / x;
+}
+
 // Parallel cyclic reduction, one row per lane (rows >= N are identity rows
 // a = c = d = 0, b = 1).  Returns x with a x_{l-1} + b x_l + c x_{l+1} = d.
 template <int G, typename JT>
 __device__ __forceinline__ JT pcr1(int gl, JT a, JT b, JT c, JT d) {
 #pragma unroll
     for (int s = 1; s < G; s <<= 1) {
-        JT ib = (JT)1 / b;
+        JT ib = recip(b);
         JT am = g_up<G>(a, s), cm = g_up<G>(c, s), dm = g_up<G>(d, s), ibm = g_up<G>(ib, s);
         JT ap = g_down<G>(a, s), cp = g_down<G>(c, s), dp = g_down<G>(d, s), ibp = g_down<G>(ib, s);
         if (gl < s) { am = 0; cm = 0; dm = 0; ibm = 1; }
@@ -69,7 +82,10 @@ __device__ __forceinline__ JT pcr1(int gl, JT a, JT b, JT c, JT d) {
         a = na;
         c = nc;
     }
-    return d / b;
+    if constexpr (std::is_same<JT, float>::value)
+        return d * recip(b);
+    else
+        return d / b;
 }
 
 // Two rows per lane (rows 2l, 2l+1): one cyclic-reduction level folds the
@@ -82,7 +98,11 @@ __device__ __forceinline__ void pcr2(int gl, const JT* a, const JT* b, const JT*
     JT a2 = g_down<G>(a[0], 1), b2 = g_down<G>(b[0], 1), c2 = g_down<G>(c[0], 1);
     JT d2 = g_down<G>(d[0], 1);
     if (gl + 1 >= G) { a2 = 0; b2 = 1; c2 = 0; d2 = 0; }
-    JT al = -a[1] / b[0], ga = -c[1] / b2;
+    JT al = -a[1] * recip(b[0]), ga = -c[1] * recip(b2);
+    if constexpr (!std::is_same<JT, float>::value) {
+        al = -a[1] / b[0];
+        ga = -c[1] / b2;
+    }
     JT oa = al * a[0];
     JT oc = ga * c2;
     JT ob = b[1] + al * c[0] + ga * a2;
@@ -91,7 +111,10 @@ __device__ __forceinline__ void pcr2(int gl, const JT* a, const JT* b, const JT*
     JT xm = g_up<G>(xo, 1);  // x_{2l-1}
     if (gl == 0) xm = 0;
     x[1] = xo;
-    x[0] = (d[0] - a[0] * xm - c[0] * xo) / b[0];
+    if constexpr (std::is_same<JT, float>::value)
+        x[0] = (d[0] - a[0] * xm - c[0] * xo) * recip(b[0]);
+    else
+        x[0] = (d[0] - a[0] * xm - c[0] * xo) / b[0];
 }
 
 // Factor once, solve many: the multipliers of pcr1's elimination depend on
@@ -188,7 +211,7 @@ __device__ __forceinline__ void pcr_solve(int gl, JT (&a)[R], JT (&b)[R], JT (&c
         for (int j = 0; j < M; ++j) {
             const int k = j + 1;
             JT den = b[k] - (j > 0 ? a[k] * cp[j - 1] : (JT)0);
-            JT inv = (JT)1 / den;
+            JT inv = recip(den);
             cp[j] = c[k] * inv;
             al[j] = (d[k] - (j > 0 ? a[k] * al[j - 1] : (JT)0)) * inv;
             be[j] = ((j == 0 ? -a[1] : (JT)0) - (j > 0 ? a[k] * be[j - 1] : (JT)0)) * inv;
