@@ -15,6 +15,7 @@ namespace clm {
 struct EHPar {
     VG v;
     double nu, om, qz, gr, k_dry, k_satf, k_satu, rhoc_ds;
+    double lk_u, lk_f;  // log κ_sat_unfrozen, log κ_sat_frozen
 };
 
 __device__ __forceinline__ EHPar load_eh(const double* __restrict__ par, int64_t ns, int64_t s) {
@@ -28,6 +29,8 @@ __device__ __forceinline__ EHPar load_eh(const double* __restrict__ par, int64_t
     e.k_satf = par[P_KAPPA_SAT_FROZEN * ns + s];
     e.k_satu = par[P_KAPPA_SAT_UNFROZEN * ns + s];
     e.rhoc_ds = par[P_RHOC_DS * ns + s];
+    e.lk_u = clm_log(e.k_satu);
+    e.lk_f = clm_log(e.k_satf);
     return e;
 }
 
@@ -39,12 +42,13 @@ struct EHAux {
 __device__ __forceinline__ EHAux eh_cell_aux(const EHPar& e, double vl, double ti, double re) {
     EHAux a;
     a.thl = liq_fraction(e.v, vl, e.nu - ti);
-    a.kap = thermal_cond(a.thl, ti, e.nu, e.om, e.qz, e.gr, e.k_dry, e.k_satu, e.k_satf);
+    a.kap = thermal_cond(a.thl, ti, e.nu, e.om, e.qz, e.gr, e.k_dry, e.k_satu, e.k_satf,
+                         e.lk_u, e.lk_f);
     a.T = temperature(re, ti, heat_capacity(a.thl, ti, e.rhoc_ds));
     double fi = ti / (a.thl + ti - e.v.theta_r);
-    double imp = pow(10.0, -OMEGA * fi);                  // impedance_factor
-    double visc = exp(GAMMA_VISC * (a.T - GAMMA_T_REF));  // viscosity_factor
-    a.K = imp * visc * hyd_cond(e.v, eff_sat(e.v, e.nu, vl));
+    // impedance_factor 10^(-Ω f_i) times viscosity_factor e^(γ(T - γT_ref)), one exp
+    double impvisc = exp(-OMEGA * LN10 * fi + GAMMA_VISC * (a.T - GAMMA_T_REF));
+    a.K = impvisc * hyd_cond(e.v, eff_sat(e.v, e.nu, vl));
     a.psi = psi_dpsi(e.v, e.nu - ti, vl).psi;
     return a;
 }
@@ -70,8 +74,8 @@ __device__ __forceinline__ double surface_weight(double vl, double ti, double nu
 }
 __device__ __forceinline__ double eh_infil_capacity(double Ksat, double thr, double thl_top,
                                                     double ti_top, double T_top) {
-    return -Ksat * pow(10.0, -OMEGA * (ti_top / (thl_top + ti_top - thr))) *
-           exp(GAMMA_VISC * (T_top - GAMMA_T_REF));
+    return -Ksat * exp(-OMEGA * LN10 * (ti_top / (thl_top + ti_top - thr)) +
+                       GAMMA_VISC * (T_top - GAMMA_T_REF));
 }
 // top fluxes from the infiltration: (water, heat)
 __device__ __forceinline__ void eh_atmos_fluxes(const Surface& s, int64_t i, double infil,

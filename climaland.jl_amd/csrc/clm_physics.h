@@ -37,6 +37,7 @@ constexpr double KERSTEN_B = 18.3;
 constexpr double OMEGA = 7.0;
 constexpr double GAMMA_VISC = 2.64e-2;
 constexpr double GAMMA_T_REF = 288.0;
+constexpr double LN10 = 2.302585092994046;  // ln 10, impedance 10^x = e^(x ln 10)
 
 constexpr double DEPS = 2.220446049250313e-16;       // eps(Float64)
 constexpr double SQRT_EPS = 1.4901161193847656e-08;  // sqrt(eps(Float64))
@@ -45,6 +46,68 @@ constexpr double SQRT_EPS = 1.4901161193847656e-08;  // sqrt(eps(Float64))
 // iterate never passes `err < tol`)
 __device__ __forceinline__ double nan_max(double a, double b) {
     return (b > a || b != b) ? b : a;
+}
+
+// ---- logarithm and powers --------------------------------------------------
+// Natural logarithm within about one ulp in ~40 instructions (the library's
+// double-double routine takes ~60).  With x = 2^e m, m in [√½, √2),
+// f = m - 1 (exact) and s = f / (2 + f):
+//   log(1+f) = 2 atanh(s) = f - (f²/2 - s (f²/2 + R)),
+//   R = Σ_{k=1..10} 2 s^(2k) / (2k+1)
+// (the atanh series; |s| <= 0.1716, so the omitted tail is below 2^-60 of
+// the result), and e·ln2 is added in two parts: ln2_hi carries 42
+// significant bits, so e·ln2_hi is exact for every binary64 exponent.
+// s enters only the correction term, so a reciprocal refined by two Newton
+// steps replaces the IEEE division.  log(0) = -inf, log(+inf) = +inf,
+// log(x < 0) = log(NaN) = NaN.
+__host__ __device__ __forceinline__ double clm_log(double x) {
+    constexpr double LN2_HI = 0.6931471805598903;     // 0x3fe62e42fefa3800
+    constexpr double LN2_LO = 5.497923018708371e-14;  // ln 2 - LN2_HI
+    constexpr double SQRT_HALF = 0.7071067811865476;
+#if defined(__HIP_DEVICE_COMPILE__)
+    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+#else
+    int e;
+    double m = frexp(x, &e);
+#endif
+    if (m < SQRT_HALF) {
+        m += m;
+        e -= 1;
+    }
+    const double f = m - 1.0;
+    const double d = 2.0 + f;
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    const double s = f * r;
+#else
+    const double s = f / d;
+#endif
+    const double z = s * s;
+    double p = 2.0 / 21.0;
+    p = fma(p, z, 2.0 / 19.0);
+    p = fma(p, z, 2.0 / 17.0);
+    p = fma(p, z, 2.0 / 15.0);
+    p = fma(p, z, 2.0 / 13.0);
+    p = fma(p, z, 2.0 / 11.0);
+    p = fma(p, z, 2.0 / 9.0);
+    p = fma(p, z, 2.0 / 7.0);
+    p = fma(p, z, 2.0 / 5.0);
+    p = fma(p, z, 2.0 / 3.0);
+    const double R = p * z;
+    const double hfsq = 0.5 * f * f;
+    const double de = (double)e;
+    const double y = de * LN2_HI - ((hfsq - (s * (hfsq + R) + de * LN2_LO)) - f);
+    const double sp = (x == 0.0) ? -INFINITY : ((x == INFINITY) ? x : NAN);
+    return (x > 0.0 && x < INFINITY) ? y : sp;
+}
+
+// x^y = exp(y log x) for x >= 0: relative error about |y log x| ulp, which
+// the closures' bases and exponents keep below ~50 ulp (1e-14).
+__host__ __device__ __forceinline__ double clm_pow(double x, double y) {
+    return exp(y * clm_log(x));
 }
 
 // per-column van Genuchten constants, derived once per column
@@ -99,10 +162,10 @@ __device__ __forceinline__ PsiD psi_dpsi(const VG& v, double nu_eff, double vl) 
     double S = (vs - v.theta_r) / span;
     r.S = S;
     if (S < 1.0) {
-        double lS = log(S);
+        double lS = clm_log(S);
         double a = exp(-lS * v.inv_m);
         double X = a - 1.0;
-        double lX = log(X);
+        double lX = clm_log(X);
         double X1n = exp(lX * v.inv_n);
         r.a = a;
         r.psi = -X1n * v.inv_alpha;
@@ -143,10 +206,10 @@ __device__ __forceinline__ PsiI psi_invd(const VG& v, const Span& sp, double vl)
     double S = (vs - v.theta_r) * sp.inv;
     r.S = S;
     if (S < 1.0) {
-        double lS = log(S);
+        double lS = clm_log(S);
         double a = exp(-lS * v.inv_m);
         double X = a - 1.0;
-        double lX = log(X);
+        double lX = clm_log(X);
         double X1n = exp(lX * v.inv_n);
         r.a = a;
         r.psi = -X1n * v.inv_alpha;
@@ -164,7 +227,7 @@ __device__ __forceinline__ PsiI psi_invd(const VG& v, const Span& sp, double vl)
 __device__ __forceinline__ double hyd_cond_from_a(const VG& v, double S, double a) {
     if (S < 1.0) {
         double Y = 1.0 - 1.0 / a;  // 1 - S^(1/m)
-        double Ym = exp(v.m * log(Y));
+        double Ym = exp(v.m * clm_log(Y));
         double q = 1.0 - Ym;
         return sqrt(S) * (q * q) * v.Ksat;
     }
@@ -174,21 +237,23 @@ __device__ __forceinline__ double hyd_cond_from_a(const VG& v, double S, double 
 // hydraulic_conductivity with its own saturation
 __device__ __forceinline__ double hyd_cond(const VG& v, double S) {
     if (S < 1.0) {
-        double a = exp(-log(S) * v.inv_m);
+        double a = exp(-clm_log(S) * v.inv_m);
         return hyd_cond_from_a(v, S, a);
     }
     return v.Ksat;
 }
 
-// matric_potential (:59-63)
+// matric_potential (:59-63), -((S^(-1/m) - 1) α^(-n))^(1/n), written as
+// -(S^(-1/m) - 1)^(1/n) / α
 __device__ __forceinline__ double matric(const VG& v, double S) {
-    return -pow((pow(S, -v.inv_m) - 1.0) * pow(v.alpha, -v.n), v.inv_n);
+    double X = exp(-clm_log(S) * v.inv_m) - 1.0;
+    return -exp(clm_log(X) * v.inv_n) * v.inv_alpha;
 }
 
 // inverse_matric_potential (:72-77); *bad = 1 where the reference raises
 __device__ __forceinline__ double inv_matric(const VG& v, double psi, int* bad) {
     if (psi > 0.0) *bad = 1;
-    return pow(1.0 + pow(v.alpha * fabs(psi), v.n), -v.m);
+    return clm_pow(1.0 + clm_pow(v.alpha * fabs(psi), v.n), -v.m);
 }
 
 // ---- heat (soil_heat_parameterizations.jl) --------------------------------
@@ -200,23 +265,26 @@ __device__ __forceinline__ double temperature(double rhoe, double thi, double rh
 }
 __device__ __forceinline__ double energy_liq(double T) { return RCP_L * (T - T_0); }  // :221-231
 
-// κ (:243-320): Balland–Arp Kersten number and saturated conductivity
+// κ (:243-320): Balland–Arp Kersten number and saturated conductivity.
+// Both Kersten branches are one power product Sr^p g^q (frozen: p = 1+ν_om,
+// g^q = 1), evaluated without divergence as exp(p log Sr + q log g);
+// κ_u^(θ_l/θ_w) κ_f^(θ_i/θ_w) = exp((θ_l log κ_u + θ_i log κ_f) / θ_w) with
+// the per-column logarithms lk_u, lk_f.
 __device__ __forceinline__ double thermal_cond(double thl, double thi, double nu, double om,
                                                double qz, double gr, double k_dry,
-                                               double k_satu, double k_satf) {
+                                               double k_satu, double k_satf, double lk_u,
+                                               double lk_f) {
     double Sr = (thl + thi) / nu;
-    double Ke;
-    if (thi < DEPS) {
-        double t1 = pow(1.0 + exp(-KERSTEN_B * Sr), -3.0);
-        double h = (1.0 - Sr) / 2.0;
-        Ke = pow(Sr, (1.0 + om - KERSTEN_A * qz - gr) / 2.0) *
-             pow(t1 - h * h * h, 1.0 - om);
-    } else {
-        Ke = pow(Sr, 1.0 + om);
-    }
+    const bool unfrozen = thi < DEPS;
+    double xk = 1.0 + exp(-KERSTEN_B * Sr);
+    double t1 = 1.0 / (xk * xk * xk);  // (1 + e^(-β Sr))^-3
+    double h = (1.0 - Sr) / 2.0;
+    double p = unfrozen ? (1.0 + om - KERSTEN_A * qz - gr) / 2.0 : 1.0 + om;
+    double q = unfrozen ? 1.0 - om : 0.0;
+    double g = unfrozen ? t1 - h * h * h : 1.0;
+    double Ke = exp(p * clm_log(Sr) + q * clm_log(g));
     double tw = thl + thi;
-    double ks = (tw < DEPS) ? (k_satu + k_satf) / 2.0
-                            : pow(k_satu, thl / tw) * pow(k_satf, thi / tw);
+    double ks = (tw < DEPS) ? (k_satu + k_satf) / 2.0 : exp((thl * lk_u + thi * lk_f) / tw);
     return Ke * ks + (1.0 - Ke) * k_dry;
 }
 
@@ -231,7 +299,7 @@ __device__ __forceinline__ double phase_change(const VG& v, double nu, double th
     double psiw0 = matric(v, Sw);
     double Tf = fmax(T_FREEZE * exp(GRAV * psiw0 / LH_F0), 1.0);
     double H = (Tf - T > DEPS) ? 1.0 : 0.0;  // heaviside, utils.jl:93-99
-    double psiT = LH_F0 / GRAV * log(T / Tf) * H;
+    double psiT = LH_F0 / GRAV * clm_log(T / Tf) * H;
     double thstar = inv_matric(v, psiw0 + psiT, bad) * (nu - v.theta_r) + v.theta_r;
     return (thl - thstar) / tau;
 }
