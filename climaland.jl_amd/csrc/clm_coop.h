@@ -50,6 +50,38 @@ __device__ __forceinline__ double g_max(double v) {
     for (int o = G / 2; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, G));
     return v;
 }
+// group max of 32-bit values: DPP within rows of 16 lanes (quad swaps, then
+// the half-row and row mirrors), lane permutes across rows
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int G>
+__device__ __forceinline__ uint32_t g_umax(uint32_t v) {
+    if constexpr (G >= 2) v = max(v, dpp_u32<0xB1>(v));   // quad_perm [1,0,3,2]
+    if constexpr (G >= 4) v = max(v, dpp_u32<0x4E>(v));   // quad_perm [2,3,0,1]
+    if constexpr (G >= 8) v = max(v, dpp_u32<0x141>(v));  // row_half_mirror
+    if constexpr (G >= 16) v = max(v, dpp_u32<0x140>(v)); // row_mirror
+    if constexpr (G >= 32) v = max(v, (uint32_t)__shfl_xor(v, 16, G));
+    if constexpr (G >= 64) v = max(v, (uint32_t)__shfl_xor(v, 32, G));
+    return v;
+}
+
+// Newton's convergence test on the group's max |Δ| without an FP64
+// reduction: the high words of non-negative doubles are ordered like the
+// values, so the max of the high words decides `max < tol` unless it equals
+// tol's high word, where the exact FP64 max is taken (a group-uniform branch).
+// Returns NaN when some |Δ| is not finite, the exact max in the tie, and
+// otherwise 0 or tol: a finite value on the same side of tol as the max.
+template <int G>
+__device__ __forceinline__ double g_err(double e, double tol) {
+    const uint32_t h = (uint32_t)((uint64_t)__double_as_longlong(e) >> 32) & 0x7fffffffu;
+    const uint32_t th = (uint32_t)((uint64_t)__double_as_longlong(tol) >> 32) & 0x7fffffffu;
+    const uint32_t H = g_umax<G>(h);
+    if (H >= 0x7ff00000u) return NAN;
+    if (H == th) return g_max<G>(e);
+    return H < th ? 0.0 : tol;
+}
 
 // Reciprocal of a pivot: exact division in FP64; in the FP32 solve of the
 // mixed form (an inexact Newton whose FP64 residual decides convergence) the
@@ -64,21 +96,60 @@ __device__ __forceinline__ JT recip(JT x) {
         return (JT)1 / x;
 }
 
+// Neighbour exchange of the tridiagonal solves through LDS.  Each group of
+// G lanes owns 2G consecutive entries of its wave's LDS slice: its lanes at
+// [G/2, 3G/2) and identity rows (a = c = d = 0, b = 1/b = 1) in the G/2
+// entries on either side, so a read at distance s <= G/2 past either end of
+// the column returns an identity row with no select.  One 4-wide entry per
+// exchange replaces four lane permutes and their boundary selects.  Only the
+// owning wave touches its slice, and a wave's LDS operations complete in
+// issue order: the wave barriers keep the compiler from moving the accesses
+// across each other, which is all the ordering needed.
+template <typename JT>
+struct alignas(4 * sizeof(JT)) Q4 {
+    JT x, y, z, w;
+};
+
+template <int G, typename JT>
+struct Xch {
+    static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group of 2..64 lanes");
+    static constexpr int kWaveEntries = 128;  // (64 / G) groups x 2G entries
+    Q4<JT>* e;                                // this lane's entry
+    int gl;
+    __device__ __forceinline__ Xch(Q4<JT>* block_slices, int gl_) : gl(gl_) {
+        const int lane = threadIdx.x & 63;
+        Q4<JT>* seg = block_slices + (threadIdx.x >> 6) * kWaveEntries + (lane / G) * (2 * G);
+        e = seg + G / 2 + gl;
+        Q4<JT>* pad = (gl < G / 2) ? seg + gl : seg + G + gl;
+        *pad = Q4<JT>{(JT)0, (JT)0, (JT)0, (JT)1};
+    }
+    __device__ __forceinline__ void put(JT x, JT y, JT z, JT w) {
+        __builtin_amdgcn_wave_barrier();
+        *e = Q4<JT>{x, y, z, w};
+        __builtin_amdgcn_wave_barrier();
+    }
+    // one component only (the readers of this exchange use that slot alone)
+    template <int SLOT>
+    __device__ __forceinline__ void put1(JT v) {
+        __builtin_amdgcn_wave_barrier();
+        (&e->x)[SLOT] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __device__ __forceinline__ Q4<JT> at(int off) const { return e[off]; }
+};
+
 // Parallel cyclic reduction, one row per lane (rows >= N are identity rows
 // a = c = d = 0, b = 1).  Returns x with a x_{l-1} + b x_l + c x_{l+1} = d.
 template <int G, typename JT>
-__device__ __forceinline__ JT pcr1(int gl, JT a, JT b, JT c, JT d) {
+__device__ __forceinline__ JT pcr1(Xch<G, JT>& xc, JT a, JT b, JT c, JT d) {
 #pragma unroll
     for (int s = 1; s < G; s <<= 1) {
-        JT ib = recip(b);
-        JT am = g_up<G>(a, s), cm = g_up<G>(c, s), dm = g_up<G>(d, s), ibm = g_up<G>(ib, s);
-        JT ap = g_down<G>(a, s), cp = g_down<G>(c, s), dp = g_down<G>(d, s), ibp = g_down<G>(ib, s);
-        if (gl < s) { am = 0; cm = 0; dm = 0; ibm = 1; }
-        if (gl + s >= G) { ap = 0; cp = 0; dp = 0; ibp = 1; }
-        JT al = -a * ibm, ga = -c * ibp;
-        JT na = al * am, nc = ga * cp;
-        b = b + al * cm + ga * ap;
-        d = d + al * dm + ga * dp;
+        xc.put(a, c, d, recip(b));
+        const Q4<JT> m = xc.at(-s), p = xc.at(s);
+        JT al = -a * m.w, ga = -c * p.w;
+        JT na = al * m.x, nc = ga * p.y;
+        b = b + al * m.y + ga * p.x;
+        d = d + al * m.z + ga * p.z;
         a = na;
         c = nc;
     }
@@ -90,26 +161,29 @@ __device__ __forceinline__ JT pcr1(int gl, JT a, JT b, JT c, JT d) {
 
 // Two rows per lane (rows 2l, 2l+1): one cyclic-reduction level folds the
 // even rows into the odd ones, PCR solves the odd rows across lanes, the
-// even rows are recovered by back substitution.
+// even rows are recovered by back substitution.  The next lane's even row
+// (a, c, d, b) comes through an exchange entry, whose identity padding
+// (b in the fourth slot) is again the row past the column's top.
 template <int G, typename JT>
-__device__ __forceinline__ void pcr2(int gl, const JT* a, const JT* b, const JT* c, const JT* d,
-                                     JT* x) {
-    // even row of the next lane (row 2l+2)
-    JT a2 = g_down<G>(a[0], 1), b2 = g_down<G>(b[0], 1), c2 = g_down<G>(c[0], 1);
-    JT d2 = g_down<G>(d[0], 1);
-    if (gl + 1 >= G) { a2 = 0; b2 = 1; c2 = 0; d2 = 0; }
-    JT al = -a[1] * recip(b[0]), ga = -c[1] * recip(b2);
-    if constexpr (!std::is_same<JT, float>::value) {
+__device__ __forceinline__ void pcr2(Xch<G, JT>& xc, const JT* a, const JT* b, const JT* c,
+                                     const JT* d, JT* x) {
+    xc.put(a[0], c[0], d[0], b[0]);
+    const Q4<JT> n = xc.at(1);  // row 2l+2
+    JT al, ga;
+    if constexpr (std::is_same<JT, float>::value) {
+        al = -a[1] * recip(b[0]);
+        ga = -c[1] * recip(n.w);
+    } else {
         al = -a[1] / b[0];
-        ga = -c[1] / b2;
+        ga = -c[1] / n.w;
     }
     JT oa = al * a[0];
-    JT oc = ga * c2;
-    JT ob = b[1] + al * c[0] + ga * a2;
-    JT od = d[1] + al * d[0] + ga * d2;
-    JT xo = pcr1<G, JT>(gl, oa, ob, oc, od);
-    JT xm = g_up<G>(xo, 1);  // x_{2l-1}
-    if (gl == 0) xm = 0;
+    JT oc = ga * n.y;
+    JT ob = b[1] + al * c[0] + ga * n.x;
+    JT od = d[1] + al * d[0] + ga * n.z;
+    JT xo = pcr1<G, JT>(xc, oa, ob, oc, od);
+    xc.template put1<0>(xo);
+    const JT xm = xc.at(-1).x;  // x_{2l-1}; 0 below the column
     x[1] = xo;
     if constexpr (std::is_same<JT, float>::value)
         x[0] = (d[0] - a[0] * xm - c[0] * xo) * recip(b[0]);
@@ -131,31 +205,27 @@ struct Pcr1Factor {
     static constexpr int L = Log2<G>::v;
     JT al[L], ga[L];
     JT ib;
-    __device__ __forceinline__ void factor(int gl, JT a, JT b, JT c) {
+    __device__ __forceinline__ void factor(Xch<G, JT>& xc, JT a, JT b, JT c) {
         int lev = 0;
 #pragma unroll
         for (int s = 1; s < G; s <<= 1, ++lev) {
-            JT ibv = (JT)1 / b;
-            JT am = g_up<G>(a, s), cm = g_up<G>(c, s), ibm = g_up<G>(ibv, s);
-            JT ap = g_down<G>(a, s), cp = g_down<G>(c, s), ibp = g_down<G>(ibv, s);
-            if (gl < s) { am = 0; cm = 0; ibm = 1; }
-            if (gl + s >= G) { ap = 0; cp = 0; ibp = 1; }
-            JT l = -a * ibm, g = -c * ibp;
+            xc.put(a, c, (JT)0, (JT)1 / b);
+            const Q4<JT> m = xc.at(-s), p = xc.at(s);
+            JT l = -a * m.w, g = -c * p.w;
             al[lev] = l;
             ga[lev] = g;
-            b = b + l * cm + g * ap;
-            a = l * am;
-            c = g * cp;
+            b = b + l * m.y + g * p.x;
+            a = l * m.x;
+            c = g * p.y;
         }
         ib = (JT)1 / b;
     }
-    __device__ __forceinline__ JT solve(int gl, JT d) const {
+    __device__ __forceinline__ JT solve(Xch<G, JT>& xc, JT d) const {
         int lev = 0;
 #pragma unroll
         for (int s = 1; s < G; s <<= 1, ++lev) {
-            JT dm = g_up<G>(d, s), dp = g_down<G>(d, s);
-            if (gl < s) dm = 0;
-            if (gl + s >= G) dp = 0;
+            xc.template put1<2>(d);
+            const JT dm = xc.at(-s).z, dp = xc.at(s).z;
             d = d + al[lev] * dm + ga[lev] * dp;
         }
         return d * ib;
@@ -168,22 +238,23 @@ template <int G, typename JT>
 struct Pcr2Factor {
     JT al, ga, a0, c0, ib0;
     Pcr1Factor<G, JT> odd;
-    __device__ __forceinline__ void factor(int gl, const JT* a, const JT* b, const JT* c) {
-        JT a2 = g_down<G>(a[0], 1), b2 = g_down<G>(b[0], 1), c2 = g_down<G>(c[0], 1);
-        if (gl + 1 >= G) { a2 = 0; b2 = 1; c2 = 0; }
+    __device__ __forceinline__ void factor(Xch<G, JT>& xc, const JT* a, const JT* b,
+                                           const JT* c) {
+        xc.put(a[0], c[0], (JT)0, b[0]);
+        const Q4<JT> n = xc.at(1);
         ib0 = (JT)1 / b[0];
         al = -a[1] * ib0;
-        ga = -c[1] / b2;
+        ga = -c[1] / n.w;
         a0 = a[0];
         c0 = c[0];
-        odd.factor(gl, al * a[0], b[1] + al * c[0] + ga * a2, ga * c2);
+        odd.factor(xc, al * a[0], b[1] + al * c[0] + ga * n.x, ga * n.y);
     }
-    __device__ __forceinline__ void solve(int gl, const JT* d, JT* x) const {
-        JT d2 = g_down<G>(d[0], 1);
-        if (gl + 1 >= G) d2 = 0;
-        JT xo = odd.solve(gl, d[1] + al * d[0] + ga * d2);
-        JT xm = g_up<G>(xo, 1);
-        if (gl == 0) xm = 0;
+    __device__ __forceinline__ void solve(Xch<G, JT>& xc, const JT* d, JT* x) const {
+        xc.template put1<2>(d[0]);
+        const JT d2 = xc.at(1).z;
+        JT xo = odd.solve(xc, d[1] + al * d[0] + ga * d2);
+        xc.template put1<0>(xo);
+        const JT xm = xc.at(-1).x;
         x[1] = xo;
         x[0] = (d[0] - a0 * xm - c0 * xo) * ib0;
     }
@@ -198,12 +269,12 @@ struct Pcr2Factor {
 // complement of a diagonally dominant matrix, solved by pcr2.  Interior
 // unknowns follow from the formula above.
 template <int G, int R, typename JT>
-__device__ __forceinline__ void pcr_solve(int gl, JT (&a)[R], JT (&b)[R], JT (&c)[R], JT (&d)[R],
-                                          JT (&x)[R]) {
+__device__ __forceinline__ void pcr_solve(Xch<G, JT>& xc, JT (&a)[R], JT (&b)[R], JT (&c)[R],
+                                          JT (&d)[R], JT (&x)[R]) {
     if constexpr (R == 1) {
-        x[0] = pcr1<G, JT>(gl, a[0], b[0], c[0], d[0]);
+        x[0] = pcr1<G, JT>(xc, a[0], b[0], c[0], d[0]);
     } else if constexpr (R == 2) {
-        pcr2<G, JT>(gl, a, b, c, d, x);
+        pcr2<G, JT>(xc, a, b, c, d, x);
     } else {
         constexpr int M = R - 2;  // interior rows 1..R-2
         JT cp[M], al[M], be[M], ga[M];
@@ -232,7 +303,7 @@ __device__ __forceinline__ void pcr_solve(int gl, JT (&a)[R], JT (&b)[R], JT (&c
         rb[1] = b[R - 1] + a[R - 1] * ga[M - 1];
         rc[1] = c[R - 1];
         rd[1] = d[R - 1] - a[R - 1] * al[M - 1];
-        pcr2<G, JT>(gl, ra, rb, rc, rd, rx);
+        pcr2<G, JT>(xc, ra, rb, rc, rd, rx);
         x[0] = rx[0];
         x[R - 1] = rx[1];
 #pragma unroll
@@ -276,6 +347,8 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
     const int gl = id.gl;
+    __shared__ Q4<JT> xmem[4 * Xch<G, JT>::kWaveEntries];  // 256 threads: 4 waves
+    Xch<G, JT> xc(xmem, gl);
     const int col = bin_cols[id.grp];
     const int64_t base = col_ptr[col];
     const int N = (int)(col_ptr[col + 1] - base);
@@ -424,7 +497,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
                 d[k] = 0;
             }
         }
-        pcr_solve<G, R, JT>(gl, a, b, c, d, y);
+        pcr_solve<G, R, JT>(xc, a, b, c, d, y);
         double e = 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -432,7 +505,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
             x[k] -= dx;
             e = nan_max(e, fabs(dx));
         }
-        err = g_max<G>(e);
+        err = g_err<G>(e, tol);
         conv = (tol > 0.0) && (err < tol);
         if (conv || !isfinite(err)) break;
     }
@@ -472,6 +545,8 @@ __global__ __launch_bounds__(256) void k_eh_coop(
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
     const int gl = id.gl;
+    __shared__ Q4<JT> xmem[4 * Xch<G, JT>::kWaveEntries];  // 256 threads: 4 waves
+    Xch<G, JT> xc(xmem, gl);
     const int col = bin_cols[id.grp];
     const int64_t base = col_ptr[col];
     const int N = (int)(col_ptr[col + 1] - base);
@@ -609,6 +684,16 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         wK[k] = (r == 0) ? 0.0 : Kf[k] * idzf[k];
         wk[k] = (r == 0) ? 0.0 : kf[k] * idzf[k];
     }
+    // the same conductances on the face above each row (zero above the top row)
+    double wKa[R], wka[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const bool last = gl * R + k >= N - 1;
+        double wKn = (k + 1 < R) ? wK[k + 1] : g_down<G>(wK[0], 1);
+        double wkn = (k + 1 < R) ? wk[k + 1] : g_down<G>(wk[0], 1);
+        wKa[k] = last ? 0.0 : wKn;
+        wka[k] = last ? 0.0 : wkn;
+    }
 
     // the ρe block of W is fixed for the whole step (lagged κ and ρc_s): with
     // one or two rows per lane it is reduced once here
@@ -618,13 +703,9 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         JT ea[R], eb[R], ec[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            int r = gl * R + k;
-            bool last = r >= N - 1;
-            double wkn = (k + 1 < R) ? wk[k + 1] : g_down<G>(wk[0], 1);
-            double wku = last ? 0.0 : wkn;
             ea[k] = (JT)(dt * wk[k]);
-            ec[k] = (JT)(dt * wku);
-            eb[k] = (JT)(-dt * (wk[k] + wku) - dzc[k] * rcs[k]);
+            ec[k] = (JT)(dt * wka[k]);
+            eb[k] = (JT)(-dt * (wk[k] + wka[k]) - dzc[k] * rcs[k]);
             if (!on[k]) {
                 ea[k] = 0;
                 eb[k] = 1;
@@ -632,9 +713,9 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             }
         }
         if constexpr (R == 1)
-            efac.factor(gl, ea[0], eb[0], ec[0]);
+            efac.factor(xc, ea[0], eb[0], ec[0]);
         else
-            efac.factor(gl, ea, eb, ec);
+            efac.factor(xc, ea, eb, ec);
     }
     const bool warm = gv != nullptr;
     int kit = 0, kit_total = 0;
@@ -702,10 +783,9 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             bool last = r >= N - 1;
             // shuffles run on every lane of the group (a lane that skips one
             // would read as zero to its neighbour); boundary selects come after
-            double wKu = (k + 1 < R) ? wK[k + 1] : g_down<G>(wK[0], 1);
             double Fu = (k + 1 < R) ? F[k + 1] : g_down<G>(F[0], 1);
             double Gu = (k + 1 < R) ? Gf[k + 1] : g_down<G>(Gf[0], 1);
-            double wu = last ? 0.0 : wKu;
+            const double wu = wKa[k];
             if (last) {
                 Fu = topw;
                 Gu = toph;
@@ -727,7 +807,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
                 d[k] = 0;
             }
         }
-        pcr_solve<G, R, JT>(gl, a, b, c, d, y);  // y = D ∘ Δϑ
+        pcr_solve<G, R, JT>(xc, a, b, c, d, y);  // y = D ∘ Δϑ
         // ρe block right-hand side: Δz_c (R_ρe - W_ρϑ Δϑ), with
         // Δz_c W_ρϑ Δϑ = dt M_E y - Δz_c Δϑ (the reference's "- I", :554-556)
         double yv[R], dv[R];
@@ -736,22 +816,24 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             yv[k] = on[k] ? (double)y[k] : 0.0;
             dv[k] = on[k] ? yv[k] * iD[k] : 0.0;
         }
+        // y of the row below the lane's first row and above its last (0 past
+        // the column's ends, where the conductances are 0)
+        xc.put(on[R - 1] ? y[R - 1] : (JT)0, on[0] ? y[0] : (JT)0, (JT)0, (JT)0);
+        const double yb = (double)xc.at(-1).x, ya = (double)xc.at(1).y;
         double e2 = 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             int r = gl * R + k;
             bool last = r >= N - 1;
             double wEn = (k + 1 < R) ? wE[k + 1] : g_down<G>(wE[0], 1);
-            double wkn = (k + 1 < R) ? wk[k + 1] : g_down<G>(wk[0], 1);
-            double ym = below<G, R>(yv, k);
-            double yp = (k + 1 < R) ? yv[k + 1] : g_down<G>(yv[0], 1);
+            double ym = k > 0 ? yv[k - 1] : yb;
+            double yp = (k + 1 < R) ? yv[k + 1] : ya;
             double wEu = last ? 0.0 : wEn;
-            double wku = last ? 0.0 : wkn;
             double MEy = wE[k] * (ym - yv[k]) + wEu * (yp - yv[k]);
             double rhs = dRe[k] - (dt * MEy - dzc[k] * dv[k]);
             a[k] = (JT)(dt * wk[k]);
-            c[k] = (JT)(dt * wku);
-            b[k] = (JT)(-dt * (wk[k] + wku) - dzc[k] * rcs[k]);
+            c[k] = (JT)(dt * wka[k]);
+            b[k] = (JT)(-dt * (wk[k] + wka[k]) - dzc[k] * rcs[k]);
             d[k] = (JT)rhs;
             if (!on[k]) {
                 a[k] = 0;
@@ -763,11 +845,11 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             e2 = nan_max(e2, fabs(dv[k]));
         }
         if constexpr (R == 1)
-            y[0] = efac.solve(gl, d[0]);  // y = Δρe / ρc_s
+            y[0] = efac.solve(xc, d[0]);  // y = Δρe / ρc_s
         else if constexpr (R == 2)
-            efac.solve(gl, d, y);
+            efac.solve(xc, d, y);
         else
-            pcr_solve<G, R, JT>(gl, a, b, c, d, y);
+            pcr_solve<G, R, JT>(xc, a, b, c, d, y);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             double yr = on[k] ? (double)y[k] : 0.0;
@@ -780,7 +862,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
             Pp[k] = pd.psi;
             iD[k] = pd.invD;
         }
-        err = g_max<G>(e2);
+        err = g_err<G>(e2, tol);
         conv = (tol > 0.0) && (err < tol);
         if (conv || !isfinite(err)) break;
     }
