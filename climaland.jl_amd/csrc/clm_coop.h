@@ -532,8 +532,15 @@ __global__ __launch_bounds__(256) void k_rre_coop(
 // In update_cache!(u_n): the infiltration (TOPMODEL or the precipitation),
 // the top fluxes, and the subsurface sinks held through the step; in the
 // explicit stage the sublimation source on θ_i and ∫F_w.
+// Blocks per CU the EnergyHydrology kernels are compiled for: the mixed form
+// fits four waves per SIMD (<= 128 VGPRs, a few spills outside the Newton
+// loop), which hides more of the latency of the dependent solve levels than
+// the three its register need would give; the FP64 form spills at four.
+template <typename JT>
+constexpr int kEhMinBlocks = std::is_same<JT, float>::value ? 4 : 1;
+
 template <int G, int R, typename JT, bool ATM = false>
-__global__ __launch_bounds__(256) void k_eh_coop(
+__global__ __launch_bounds__(256, kEhMinBlocks<JT>) void k_eh_coop(
     const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
     double* __restrict__ xvg, double* __restrict__ xig, double* __restrict__ xeg,
@@ -587,7 +594,7 @@ __global__ __launch_bounds__(256) void k_eh_coop(
         double F = 0.0;
         if (on[k]) {
             double tau = 3.0 * heat_capacity(a.thl, t, e.rhoc_ds) * dzc[k] * dzc[k] / a.kap;
-            F = phase_change(e.v, e.nu, a.thl, t, a.T, tau, &bad);
+            F = phase_change_psi(e.v, e.nu, a.thl, t, a.T, tau, a.psiw0, &bad);
         }
         u0v[k] = v + dt * (-F);
         if (u0out && on[k]) u0out[base + r] = u0v[k];

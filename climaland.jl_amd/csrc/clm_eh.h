@@ -34,9 +34,10 @@ __device__ __forceinline__ EHPar load_eh(const double* __restrict__ par, int64_t
     return e;
 }
 
-// explicit-stage cache of one cell (energy_hydrology.jl:745-791)
+// explicit-stage cache of one cell (energy_hydrology.jl:745-791), and ψ_w0
+// of its phase change (matric_total)
 struct EHAux {
-    double thl, kap, T, K, psi;
+    double thl, kap, T, K, psi, psiw0;
 };
 
 __device__ __forceinline__ EHAux eh_cell_aux(const EHPar& e, double vl, double ti, double re) {
@@ -48,8 +49,19 @@ __device__ __forceinline__ EHAux eh_cell_aux(const EHPar& e, double vl, double t
     double fi = ti / (a.thl + ti - e.v.theta_r);
     // impedance_factor 10^(-Ω f_i) times viscosity_factor e^(γ(T - γT_ref)), one exp
     double impvisc = exp(-OMEGA * LN10 * fi + GAMMA_VISC * (a.T - GAMMA_T_REF));
-    a.K = impvisc * hyd_cond(e.v, eff_sat(e.v, e.nu, vl));
-    a.psi = psi_dpsi(e.v, e.nu - ti, vl).psi;
+    if (__all(ti == 0.0)) {
+        // no ice on the wave: ν_eff = ν, and the saturation of K, ψ and ψ_w0 is
+        // one S (for S < 1; ψ_w0 = ψ(1) = -0 otherwise), so they share S and
+        // S^(-1/m) -- the same expressions as the general branch, bit for bit
+        PsiD p = psi_dpsi(e.v, e.nu, vl);
+        a.K = impvisc * hyd_cond_from_a(e.v, p.S, p.a);
+        a.psi = p.psi;
+        a.psiw0 = p.S < 1.0 ? p.psi : -0.0;
+    } else {
+        a.K = impvisc * hyd_cond(e.v, eff_sat(e.v, e.nu, vl));
+        a.psi = psi_dpsi(e.v, e.nu - ti, vl).psi;
+        a.psiw0 = matric_total(e.v, e.nu, a.thl, ti);
+    }
     return a;
 }
 
@@ -433,7 +445,7 @@ __global__ __launch_bounds__(256) void k_eh_step(
         }
         double dz = zf[fb + (int64_t)kWave * (j + 1)] - zf[fb + (int64_t)kWave * j];
         double tau = 3.0 * heat_capacity(a.thl, t, e.rhoc_ds) * dz * dz / a.kap;
-        double F = phase_change(e.v, e.nu, a.thl, t, a.T, tau, &bad);
+        double F = phase_change_psi(e.v, e.nu, a.thl, t, a.T, tau, a.psiw0, &bad);
         double u0v = v + dt * (-F);
         double u0i = t + dt * ((RHO_L / RHO_I) * F);
         B.xv[c] = u0v;

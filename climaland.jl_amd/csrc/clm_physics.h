@@ -288,20 +288,37 @@ __device__ __forceinline__ double thermal_cond(double thl, double thi, double nu
     return Ke * ks + (1.0 - Ke) * k_dry;
 }
 
-// phase_change_source (:89-122) with soil_Tf_depressed (:35-52); τ from
-// thermal_time (:60-62) on the layer width
-__device__ __forceinline__ double phase_change(const VG& v, double nu, double thl, double thi,
-                                               double T, double tau, int* bad) {
+// ψ_w0 of phase_change_source (:89-100): the matric potential of the total
+// water θ_l + (ρ_i/ρ_l) θ_i at porosity ν
+__device__ __forceinline__ double matric_total(const VG& v, double nu, double thl, double thi) {
     double ttot = fmin(RHO_I / RHO_L * thi + thl, nu);
     double vs = fmax(ttot, v.thr_safe);
     double ns = fmax(nu, v.thr_safe);
     double Sw = (vs - v.theta_r) / (ns - v.theta_r);
-    double psiw0 = matric(v, Sw);
+    return matric(v, Sw);
+}
+
+// phase_change_source (:89-122) with soil_Tf_depressed (:35-52) given ψ_w0; τ
+// from thermal_time (:60-62) on the layer width.  Without ice and above the
+// depressed freezing point, θ* = inverse_matric_potential(ψ_w0)(ν-θ_r)+θ_r is
+// θ_l itself (ψ_w0 is the matric potential of θ_l), so F = 0 exactly; the
+// θ* round trip, which would return θ_l up to rounding, is skipped when that
+// holds on the whole wave.
+__device__ __forceinline__ double phase_change_psi(const VG& v, double nu, double thl,
+                                                   double thi, double T, double tau,
+                                                   double psiw0, int* bad) {
     double Tf = fmax(T_FREEZE * exp(GRAV * psiw0 / LH_F0), 1.0);
     double H = (Tf - T > DEPS) ? 1.0 : 0.0;  // heaviside, utils.jl:93-99
+    const bool still = thi == 0.0 && H == 0.0;
+    if (__all(still)) return 0.0;
     double psiT = LH_F0 / GRAV * clm_log(T / Tf) * H;
     double thstar = inv_matric(v, psiw0 + psiT, bad) * (nu - v.theta_r) + v.theta_r;
-    return (thl - thstar) / tau;
+    return still ? 0.0 : (thl - thstar) / tau;
+}
+
+__device__ __forceinline__ double phase_change(const VG& v, double nu, double thl, double thi,
+                                               double T, double tau, int* bad) {
+    return phase_change_psi(v, nu, thl, thi, T, tau, matric_total(v, nu, thl, thi), bad);
 }
 
 }  // namespace clm
