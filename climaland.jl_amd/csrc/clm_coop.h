@@ -554,7 +554,9 @@ __global__ __launch_bounds__(256, kEhMinBlocks<JT>) void k_eh_coop(
     const int gl = id.gl;
     __shared__ Q4<JT> xmem[4 * Xch<G, JT>::kWaveEntries];  // 256 threads: 4 waves
     Xch<G, JT> xc(xmem, gl);
-    const int col = bin_cols[id.grp];
+    // a 64-lane group is the whole wave: its column index, offsets and
+    // parameters are wave-uniform (scalar registers and loads)
+    const int col = G == 64 ? __builtin_amdgcn_readfirstlane(bin_cols[id.grp]) : bin_cols[id.grp];
     const int64_t base = col_ptr[col];
     const int N = (int)(col_ptr[col + 1] - base);
     const int64_t fbase = base + col;
@@ -885,7 +887,7 @@ __global__ __launch_bounds__(256, kEhMinBlocks<JT>) void k_eh_coop(
             xeg[i] = xe[k];
         }
     }
-    int bad_g = (int)g_max<G>((double)bad);
+    const uint32_t bad_g = g_umax<G>((uint32_t)bad);
     if (gl == 0) {
         // ∫F: U0 (explicit: sublimation) then the -I block with the step's fluxes
         double iw0 = ATM ? iw[col] + dt * (0.0 + -vice) : iw[col];

@@ -1819,6 +1819,8 @@ int clm_set_shape(clm_ctx* c, int32_t bucket_max_layers, int32_t G, int32_t R) {
     return CLM_OK;
 }
 
+constexpr int kTuneSteps = 4;  // consecutive steps per autotune trial
+
 int clm_autotune(clm_ctx* c, double dt, int32_t reps) {
     if (!c || !(dt > 0.0) || reps < 1) return CLM_E_ARG;
     if (!is_group(c)) return CLM_OK;
@@ -1863,13 +1865,19 @@ int clm_autotune(clm_ctx* c, double dt, int32_t reps) {
             for (int k = 0; k < reps && e == hipSuccess; ++k) {
                 e = copy_all(false);
                 if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
-                if (e == hipSuccess) {
+                // a few consecutive steps, regrouped by Newton count in between as
+                // the run is: the first step from the initial state iterates
+                // more than the run does, and shapes with several columns per
+                // wave are only as fast as the run makes them once a wave's
+                // columns have similar counts
+                for (int st = 0; st < kTuneSteps && e == hipSuccess; ++st) {
                     if (c->jac_prec == 32)
                         launch_bin_any<float>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters,
                                               current_extra(c));
                     else
                         launch_bin_any<double>(c, bn, c->d_cftw, c->d_cfth, dt, c->d_iters,
                                                current_extra(c));
+                    if (st + 1 < kTuneSteps) launch_regroup(c, c->d_iters);
                     e = hipGetLastError();
                 }
                 if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);

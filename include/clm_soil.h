@@ -270,9 +270,11 @@ int clm_get_shapes(clm_ctx* ctx, int32_t* records /* 4 x max_records */,
 /* Force a bucket's shape; (G, R) must be one of the compiled shapes. */
 int clm_set_shape(clm_ctx* ctx, int32_t bucket_max_layers, int32_t G, int32_t R);
 /* Autotuner: for every non-empty bucket, time each compiled shape that fits
- * (reps launches of one step from the current state and clm_set_forcing
- * values; the state is restored afterwards) and keep the fastest.  Invalidates
- * the captured run graph.  Allocates a temporary state copy. */
+ * (best of reps trials, each four consecutive steps from the current state
+ * and clm_set_forcing values, regrouped by Newton count between steps; the
+ * state is restored afterwards, the bins keep the regrouped column order)
+ * and keep the fastest.  Invalidates the captured run graph.  Allocates a
+ * temporary state copy. */
 int clm_autotune(clm_ctx* ctx, double dt, int32_t reps);
 
 /* Device time of the dominant kernel in the last clm_step/clm_run, per
