@@ -16,7 +16,7 @@
 // Outputs (Nout = 32 RRE, 64 EH): node values of (x_ϑ - U0_ϑ) / 1e-2 and,
 // for EH, of (x_ρe - u_ρe) / ρc_s(u) in kelvin.
 //
-// The Newton loop starts from x0 = U0 + guess and remains the judge: it
+// The Newton loop starts from x0 = U0 + guess and still decides convergence: it
 // iterates to the same tolerance, and falls back to the plain start U0 when
 // the warm-started iterate turns non-finite.
 #pragma once

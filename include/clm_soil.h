@@ -143,7 +143,7 @@ typedef struct {
     int32_t jac_precision;    /* 64: FP64 Jacobian+solve; 32: FP32 Jacobian+solve,
                                  residual/state/closures stay FP64 */
     int32_t newton_max_iters; /* NewtonsMethod(max_iters) */
-    double newton_tol;        /* <= 0: exactly max_iters iterations (no checker);
+    double newton_tol;        /* <= 0: exactly max_iters iterations (no convergence test);
                                  > 0: per-column stop when max|Δx| < tol over
                                  the cell unknowns (ρe_int scaled by 1/ρc_s) */
     int32_t solver;           /* CLM_SOLVER_*: how clm_step / clm_run solve */

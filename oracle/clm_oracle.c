@@ -2,7 +2,8 @@
  * clm_oracle.c -- CPU FP64 restatement of ClimaLand.jl's implicit
  * soil-column step.  TEST INFRASTRUCTURE ONLY: imported by tests/, by
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
- * checker / CPU baseline, never as the measured or shipped path.
+ * reference for comparison or the CPU baseline, never as the measured or
+ * shipped path.
  *
  * Restated from (ClimaLand.jl v1.11.2, paths relative to its root):
  *   src/standalone/Soil/soil_hydrology_parameterizations.jl  (vG closures)

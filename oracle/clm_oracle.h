@@ -1,7 +1,7 @@
 /*
  * clm_oracle.h -- CPU FP64 restatement of ClimaLand's implicit soil-column
  * step (RichardsModel and EnergyHydrology), used ONLY as test
- * infrastructure: the parity checker for the HIP library and the CPU
+ * infrastructure: the parity reference for the HIP library and the CPU
  * baseline timed beside it.  Nothing in the product path links this.
  *
  * Every function cites the reference file:line it restates (paths relative

@@ -1,7 +1,7 @@
 """CPU restatement of the learned warm start (features, network, loss and
 gradients, node <-> layer resampling) in numpy.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the library's
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the reference for the library's
 clm_ws_* kernels; never by the library's product path.
 
 The network mirrors the library's: BF16 inputs, weights and activations,

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU FP64 oracle (oracle/liborc.so).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke()
-and by bench.py's cpu_baseline leg, as the checker or the CPU baseline;
+and by bench.py's cpu_baseline leg, as the parity reference or the CPU baseline;
 never by the library's product path.
 """
 from __future__ import annotations

@@ -25,7 +25,7 @@ def oracle_mod():
 
 @pytest.fixture(scope="session")
 def gpu_lib():
-    """The HIP library on a GPU box: fail loudly if it is missing."""
+    """The HIP library on a machine with a GPU: fail loudly if it is missing."""
     import torch  # noqa: F401  (device presence check only)
     if not torch.cuda.is_available():
         pytest.skip("no GPU visible")

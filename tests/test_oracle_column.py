@@ -367,7 +367,7 @@ def test_converged_step_is_root_of_implicit_euler(oracle_mod, name):
 def test_warm_start_reaches_the_same_root(oracle_mod, name):
     """Newton from x0 = U0 + guess: the exact increment converges at once; a
     perturbed guess ends at the plain step's root to the Newton tolerance
-    (Newton stays the judge).  For RichardsModel a zero guess is the plain
+    (Newton still decides convergence).  For RichardsModel a zero guess is the plain
     step bit for bit; for EnergyHydrology it is not, because the warm start
     refreshes T and ψ at x0 while the plain first iteration uses those of
     u_n -- it still reaches the same root."""
