@@ -343,10 +343,17 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         sample = args.cpu_sample or min(batch.ncol, 24576 if batch.model == 1 else 16384)
         cps, el = cpu_baseline(batch_steps, steps, sample, threads)
+        # one core on a sixteenth of the sample, beside the reference's own
+        # single-process CPU figures
+        sample1 = max(256, sample // 16)
+        cps1, el1 = cpu_baseline(batch_steps, steps, sample1, 1)
         out["cpu_baseline"] = {"value": cps, "unit": "column-steps/s", "cores": threads,
                                "kind": "port",
                                "sample": f"first {sample} columns x {steps} steps of {w.name}, "
-                                         f"FP64 oracle, OpenMP over columns ({el:.1f} s)"}
+                                         f"FP64 oracle, OpenMP over columns ({el:.1f} s)",
+                               "single_core": {"value": cps1, "cores": 1,
+                                               "sample": f"first {sample1} columns x {steps} "
+                                                         f"steps, one thread ({el1:.1f} s)"}}
     print(json.dumps(out), flush=True)
 
 
