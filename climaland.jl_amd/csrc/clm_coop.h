@@ -4,7 +4,8 @@
 // of its column (level 0 = bottom).  The whole ARS111 step -- explicit stage
 // and every Newton iteration -- runs out of registers: the column's state
 // is read from HBM once and written once per step.  Neighbour values move
-// with intra-group shuffles.  Newton iterations are group-uniform, so a
+// through the wave's LDS slice in the tridiagonal solves (Xch below) and by
+// lane permutes elsewhere.  Newton iterations are group-uniform, so a
 // column that needs many iterations holds back only its own wavefront.
 //
 // Linear solve.  The modified-Picard block W = dtγ J - I with
@@ -13,7 +14,7 @@
 //     (dtγ M - diag(Δz_c / D)) y = Δz_c ∘ R,     Δ = y / D,
 // exactly the same linear system with rows scaled by Δz_c and unknowns by
 // D.  That matrix is symmetric and strictly diagonally dominant, so
-// parallel cyclic reduction (log2 G shuffle steps; one local cyclic-
+// parallel cyclic reduction (log2 G exchange steps; one local cyclic-
 // reduction level first when R = 2) is stable without pivoting.  For
 // EnergyHydrology the ρe-block unknown y_ρ = Δρe / ρc_s is exactly the
 // scaled quantity of the convergence norm.
@@ -24,8 +25,9 @@
 
 namespace clm {
 
-// Every lane of a group must execute each shuffle: a lane masked off by
-// divergent control flow reads back as zero to the lane that asks for it.
+// Lane permutes.  Every lane of a group must execute each one: a lane
+// masked off by divergent control flow reads back as zero to the lane that
+// asks for it.
 template <int G, typename T>
 __device__ __forceinline__ T g_up(T v, int d) {  // value of lane l-d in the group
     return __shfl_up(v, (unsigned)d, G);
