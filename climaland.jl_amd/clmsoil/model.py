@@ -242,6 +242,13 @@ class _SoilModel:
         return st
 
     def _state(self, Y: dict) -> L.State:
+        # the library reads / writes ncell or ncol values through each pointer
+        d = self.domain
+        for k in ("vartheta_l", "theta_i", "rho_e_int", "intF_water", "intF_energy"):
+            a = Y.get(k)
+            n = d.ncol if k.startswith("intF") else d.ncell
+            if a is not None and np.asarray(a).size != n:
+                raise ValueError(f"state field {k} must hold {n} values, got {np.asarray(a).size}")
         return L.State(*(L.dptr(Y.get(k)) for k in ("vartheta_l", "theta_i", "rho_e_int",
                                                     "intF_water", "intF_energy")))
 
@@ -587,10 +594,19 @@ class LandSimulation:
         return bool(same.value)
 
     def set_forcing_table(self, top_water, top_heat=None):
-        """Upload a [slot][col] forcing schedule (DriverUpdateCallback values)."""
+        """Upload a [slot][col] forcing schedule (DriverUpdateCallback values).
+        The library reads nslots x ncol values through plain pointers, so the
+        shapes are checked here."""
         m = self.model
+        ncol = m.domain.ncol
         tw = np.ascontiguousarray(top_water, dtype=np.float64)
-        th = None if top_heat is None else np.ascontiguousarray(top_heat, dtype=np.float64)
+        if tw.ndim != 2 or tw.shape[1] != ncol or tw.shape[0] < 1:
+            raise ValueError(f"top_water must be [nslots][{ncol}], got shape {tw.shape}")
+        th = None
+        if top_heat is not None:
+            th = np.ascontiguousarray(top_heat, dtype=np.float64)
+            if th.shape != tw.shape:
+                raise ValueError(f"top_heat must have top_water's shape {tw.shape}, got {th.shape}")
         m._check(m._lib.clm_set_forcing_table(m._ctx, tw.shape[0], L.dptr(tw), L.dptr(th)))
 
     def reset(self, Y0: dict, t0: float = 0.0):
@@ -615,7 +631,7 @@ class LandSimulation:
         m = self.model
         if upload and surface is not None:
             self.set_surface_flux_table(surface)
-        elif upload:
+        elif upload and top_water is not None:
             self.set_forcing_table(top_water, top_heat)
         self._push()
         st = L.StepStats()

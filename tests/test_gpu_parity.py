@@ -608,3 +608,68 @@ def test_autotune_keeps_results(gpu_lib):
     e = _col_max(b, rel_err(Y1["vartheta_l"], Y2["vartheta_l"]), ok)
     print("autotuned shapes", shapes, "max column diff vs lane solver", np.sort(e)[-5:], flush=True)
     assert np.median(e[ok]) <= 1e-12
+
+
+@pytest.mark.parametrize("jac", [32, 64])
+def test_results_independent_of_wave_mates(gpu_lib, jac):
+    """A column's step does not depend on which columns share its wave.
+    The ice-free-wave sharing of the saturation work and the skipped θ*
+    round trip are taken per wave (a vote over its lanes), but evaluate the
+    general branch's expressions, so the same columns in another order --
+    other wave-mates, other regrouping -- give the same bits and counts.
+    Shapes with 4 and 2 columns per wave; a mix of frozen and unfrozen
+    columns."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, model_from_batch
+    b = inputs.make_batch("eh_262144_uneven", ncol=512, nsteps=6)
+    w = b.workload
+    perm = np.random.default_rng(3).permutation(b.ncol)
+    outs = []
+    for bb in (b, b.column_subset(perm)):
+        m = model_from_batch(bb, jac_precision=jac, solver=1)
+        for mx, G, R in ((16, 16, 1), (24, 32, 1), (32, 32, 1), (40, 64, 1)):
+            m.set_shape(mx, G, R)
+        sim = LandSimulation(m, bb.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                             tol=w.newton_tol)
+        _, it = sim.solve(bb.top_water, bb.top_heat, 6, w.steps_per_slot, want_iters=True)
+        outs.append(({k: v.copy() for k, v in sim.state().items()}, np.asarray(it).copy()))
+        m.close()
+    (Y1, it1), (Y2, it2) = outs
+    cells = np.concatenate([np.arange(b.col_ptr[c], b.col_ptr[c + 1]) for c in perm])
+    for k in ("vartheta_l", "theta_i", "rho_e_int"):
+        assert np.array_equal(Y2[k], Y1[k][cells], equal_nan=True), k
+    for k in ("intF_water", "intF_energy"):
+        assert np.array_equal(Y2[k], Y1[k][perm], equal_nan=True), k
+    assert np.array_equal(it2, it1[..., perm])
+    # both kinds of wave occurred: columns with ice and columns without
+    assert (Y1["theta_i"] > 0).any() and (np.maximum.reduceat(Y1["theta_i"], b.col_ptr[:-1]) == 0).any()
+
+
+def test_forcing_and_state_shapes_are_checked(gpu_lib):
+    """The library reads forcing tables and states through plain pointers, so
+    the host mirror refuses arrays of the wrong size instead of letting the
+    library read past them; solve(None, None) keeps the uploaded table."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, model_from_batch
+    b = inputs.make_batch("eh_262144_uneven", ncol=64, nsteps=2)
+    w = b.workload
+    m = model_from_batch(b, jac_precision=64, solver=1)
+    sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters, tol=w.newton_tol)
+    with pytest.raises(ValueError):
+        sim.set_forcing_table(b.top_water[:, :10], b.top_heat[:, :10])
+    with pytest.raises(ValueError):
+        sim.set_forcing_table(b.top_water, b.top_heat[:, :10])
+    with pytest.raises(ValueError):
+        sim.set_forcing_table(b.top_water[0], b.top_heat[0])
+    bad = b.copy_state()
+    bad["vartheta_l"] = bad["vartheta_l"][:-1]
+    with pytest.raises(ValueError):
+        m._state(bad)
+    sim.set_forcing_table(b.top_water, b.top_heat)
+    sim.solve(None, None, 2, w.steps_per_slot)
+    Ya = sim.state()
+    sim.reset(b.copy_state())
+    sim.solve(b.top_water, b.top_heat, 2, w.steps_per_slot)
+    for k, v in sim.state().items():
+        assert np.array_equal(v, Ya[k], equal_nan=True), k
+    m.close()
