@@ -497,8 +497,14 @@ class NewtonWarmStart:
         self.model._check(self.model._lib.clm_ws_features(self.model._ctx, dt, L.fptr(X)))
         return X
 
-    def forward(self, X: np.ndarray) -> np.ndarray:
+    def _rows(self, X: np.ndarray) -> np.ndarray:
         X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != self.Fp:
+            raise ValueError(f"feature rows must be [M][{self.Fp}], got shape {X.shape}")
+        return X
+
+    def forward(self, X: np.ndarray) -> np.ndarray:
+        X = self._rows(X)
         Y = np.zeros((X.shape[0], self.Nout), dtype=np.float32)
         self.model._check(self.model._lib.clm_ws_forward(self.model._ctx, L.fptr(X), X.shape[0],
                                                          L.fptr(Y)))
@@ -519,9 +525,12 @@ class NewtonWarmStart:
         return ms.value
 
     def gradient(self, X: np.ndarray, T: np.ndarray, mask: np.ndarray):
-        X = np.ascontiguousarray(X, dtype=np.float32)
+        X = self._rows(X)
         T = np.ascontiguousarray(T, dtype=np.float32)
         mask = np.ascontiguousarray(mask, dtype=np.float32)
+        if T.shape != (X.shape[0], self.Nout) or mask.size != X.shape[0]:
+            raise ValueError(f"targets must be [{X.shape[0]}][{self.Nout}] and the mask "
+                             f"{X.shape[0]} long, got {T.shape} and {mask.shape}")
         g = np.zeros(self.n_params, dtype=np.float32)
         loss = np.zeros(1, dtype=np.float32)
         self.model._check(self.model._lib.clm_ws_gradient(
