@@ -206,6 +206,17 @@ class _SoilModel:
         self._col_ptr = np.ascontiguousarray(domain.col_ptr, dtype=np.int32)
         self._z_face = np.ascontiguousarray(domain.z_face, dtype=np.float64)
         self._params = np.ascontiguousarray(self._param_rows(), dtype=np.float64)
+        # the library reads ncol+1 offsets, ncell+ncol faces and nparam x ncol
+        # parameters through plain pointers
+        ncol = int(domain.ncol)
+        if (self._col_ptr.size != ncol + 1 or self._col_ptr[0] != 0
+                or np.any(np.diff(self._col_ptr) < 1)):
+            raise ValueError("col_ptr must be ncol+1 increasing offsets starting at 0")
+        if self._z_face.size != int(self._col_ptr[-1]) + ncol:
+            raise ValueError(f"z_face must hold ncell + ncol = {int(self._col_ptr[-1]) + ncol} "
+                             f"values, got {self._z_face.size}")
+        if self._params.ndim != 2 or self._params.shape[1] != ncol:
+            raise ValueError(f"parameter rows must be [nparam][{ncol}], got {self._params.shape}")
         cfg = L.Config(self.model_id, self._bottom_kind(), domain.ncol,
                        L.iptr(self._col_ptr), L.dptr(self._z_face), L.dptr(self._params),
                        self._params.shape[0], jac_precision, newton_max_iters, newton_tol,
