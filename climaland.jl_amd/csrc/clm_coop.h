@@ -53,7 +53,11 @@ __device__ __forceinline__ double g_max(double v) {
     return v;
 }
 // group max of 32-bit values: DPP within rows of 16 lanes (quad swaps, then
-// the half-row and row mirrors), lane permutes across rows
+// the half-row and row mirrors), then across rows the gfx950 lane swaps:
+// v_permlane16_swap exchanges rows 0<->1 and 2<->3, v_permlane32_swap the
+// two halves of the wave, so with v in both operands the pair holds the
+// lane's own value and that of lane l^16 (l^32) -- a VALU operation where a
+// lane permute is an LDS round trip
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
@@ -64,8 +68,14 @@ __device__ __forceinline__ uint32_t g_umax(uint32_t v) {
     if constexpr (G >= 4) v = max(v, dpp_u32<0x4E>(v));   // quad_perm [2,3,0,1]
     if constexpr (G >= 8) v = max(v, dpp_u32<0x141>(v));  // row_half_mirror
     if constexpr (G >= 16) v = max(v, dpp_u32<0x140>(v)); // row_mirror
-    if constexpr (G >= 32) v = max(v, (uint32_t)__shfl_xor(v, 16, G));
-    if constexpr (G >= 64) v = max(v, (uint32_t)__shfl_xor(v, 32, G));
+    if constexpr (G >= 32) {
+        auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = max((uint32_t)p[0], (uint32_t)p[1]);
+    }
+    if constexpr (G >= 64) {
+        auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        v = max((uint32_t)p[0], (uint32_t)p[1]);
+    }
     return v;
 }
 
