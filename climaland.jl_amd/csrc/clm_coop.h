@@ -545,14 +545,16 @@ __global__ __launch_bounds__(256) void k_rre_coop(
 // the top fluxes, and the subsurface sinks held through the step; in the
 // explicit stage the sublimation source on θ_i and ∫F_w.
 // Blocks per CU the EnergyHydrology kernels are compiled for: the mixed form
-// fits four waves per SIMD (<= 128 VGPRs, a few spills outside the Newton
-// loop), which hides more of the latency of the dependent solve levels than
-// the three its register need would give; the FP64 form spills at four.
-template <typename JT>
-constexpr int kEhMinBlocks = std::is_same<JT, float>::value ? 4 : 1;
+// with one row per lane fits four waves per SIMD (<= 128 VGPRs, a few spills
+// outside the Newton loop), which hides more of the latency of the dependent
+// solve levels than the three its register need would give.  With several
+// rows per lane (deeper columns) four waves would spill inside the loop, and
+// the FP64 form spills at four: those keep their register need.
+template <typename JT, int R>
+constexpr int kEhMinBlocks = (std::is_same<JT, float>::value && R == 1) ? 4 : 1;
 
 template <int G, int R, typename JT, bool ATM = false>
-__global__ __launch_bounds__(256, kEhMinBlocks<JT>) void k_eh_coop(
+__global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
     double* __restrict__ xvg, double* __restrict__ xig, double* __restrict__ xeg,
