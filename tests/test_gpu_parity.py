@@ -587,6 +587,48 @@ def test_every_group_shape_matches_oracle(gpu_lib, name):
         m.set_shape(buckets[0][0], 8, 7)
 
 
+def test_every_group_shape_mixed_form(gpu_lib):
+    """The mixed form (FP32 Jacobian and solve, FP64 residual) on every
+    compiled shape that fits, one converged step against the FP64 oracle at
+    the 1e-8-tolerance bar of test_eh_run_mixed_error.  The one-row-per-lane
+    FP32 kernels run at four waves per SIMD, the others at their own
+    register need: both builds are exercised."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation
+    b = inputs.make_batch("eh_262144_uneven", ncol=384)
+    w = b.workload
+    Yo, ito, well = _one_step_reference(b, w.newton_max_iters, w.newton_tol)
+    m, _ = _models(b, jac=32, solver=1)
+    menu = [(8, 1), (16, 1), (32, 1), (64, 1), (8, 2), (16, 2), (32, 2), (64, 2), (8, 3), (16, 3),
+            (32, 3), (4, 4), (8, 4), (16, 4), (32, 4), (8, 5), (16, 5), (8, 6), (8, 8), (16, 8)]
+    buckets = [(mx, n) for (mx, G, R, n) in m.shapes() if n > 0]
+    nmax = {mx: int(b.nlayers[(b.nlayers <= mx) & (b.nlayers > prev)].max())
+            for (mx, _), prev in zip(buckets, [0] + [x for x, _ in buckets[:-1]])}
+    rcs = np.repeat(b.params[12], b.nlayers) + 4.181e6 * Yo["vartheta_l"]
+    tried = 0
+    for G, R in menu:
+        fits = [mx for mx, _ in buckets if nmax[mx] <= G * R]
+        if not fits:
+            continue
+        for mx in fits:
+            m.set_shape(mx, G, R)
+        sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                             tol=w.newton_tol)
+        _, it = sim.solve(b.top_water[:1], b.top_heat[:1], 1, 1, want_iters=True)
+        Yg = sim.state()
+        in_fit = np.isin(np.searchsorted([mx for mx, _ in buckets], b.nlayers),
+                         [i for i, (mx, _) in enumerate(buckets) if mx in fits])
+        conv = ((it[0] & 0x30000) == 0) & (ito < w.newton_max_iters)
+        cm = _cell_mask(b, well & in_fit & conv & column_ok(b, Yg))
+        dv = np.abs(Yg["vartheta_l"][cm] - Yo["vartheta_l"][cm]).max()
+        de = (np.abs(Yg["rho_e_int"][cm] - Yo["rho_e_int"][cm]) / rcs[cm]).max()
+        print(f"mixed shape ({G}, {R}) on {fits}: |dϑ| {dv:.2g}, |dρe|/ρc {de:.2g} K", flush=True)
+        assert cm.sum() > 0 and dv <= 1e-6 and de <= 1e-6, (G, R, dv, de)
+        tried += 1
+    assert tried >= 6
+    m.close()
+
+
 def test_autotune_keeps_results(gpu_lib):
     from clmsoil import inputs
     from clmsoil.model import LandSimulation
