@@ -35,8 +35,8 @@ HBM_PEAK_GBS = 8000.0
 # scratch writes of the register spills (outside the Newton loop) of the
 # mixed-form kernels, which are compiled for four waves per SIMD.  Summary and
 # commands: profiles/k_eh_coop_eh_262144_uneven.md.
-MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1700.1e6, "raw": 1106.5e6},
-                    ("eh_262144_uneven", "mixed"): {"scaled": 1484.8e6, "raw": 993.9e6}}
+MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1718.7e6, "raw": 1116.7e6},
+                    ("eh_262144_uneven", "mixed"): {"scaled": 1469.1e6, "raw": 985.3e6}}
 
 # What bounds the fused step: VALU issue.  rocprofv3 counters of the step
 # kernels on 3 early steps of eh_262144_uneven, mixed form, shapes
@@ -44,8 +44,8 @@ MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1700.1e6, "raw":
 # SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes per step, SQ_INSTS_VALU per step, and
 # the kernels' time per step in the same build's trace.
 MEASURED_VALU = {"eh_262144_uneven": {"fp64_flop_per_step": 4.604e10,
-                                      "valu_instructions_per_step": 1.039e9,
-                                      "kernel_ms_per_step": 2.530}}
+                                      "valu_instructions_per_step": 1.046e9,
+                                      "kernel_ms_per_step": 2.496}}
 FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 CU_COUNT, ENGINE_GHZ = 256, 2.4
 
