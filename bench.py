@@ -7,13 +7,18 @@ workload's tolerance) over every column of the workload.  Inputs are
 synthetic (clmsoil.inputs) and resident in HBM before timing; the K steps
 run as one captured HIP graph.  Prints one JSON line.
 
-Forms: "learned" = the learned Newton warm start on top of "mixed";
-"mixed" = FP32 Jacobian + FP32 solve inside an FP64-residual Newton;
-"fp64" = the plain FP64 path.  `value` is the chosen form's throughput; the
-plain FP64 throughput on the same workload is reported beside it.  The
-learned form's network is trained first (untimed) on columns drawn from other
-seeds of the same workload: 8 plain steps of all its columns, spread evenly
-over the timed horizon, give the (state, converged step) pairs.
+Forms (EnergyHydrology): "fp64" = the plain FP64 path with the reference's
+modified-Picard Jacobian; "mixed" = that Jacobian and its solve in FP32 inside
+an FP64-residual Newton; "newton" = the mixed form with the exact Jacobian of
+the same residual (CLM_JACOBIAN_NEWTON, FP32); "learned" = the learned Newton
+warm start on top of "newton".  RichardsModel: "fp64" and "learned" (on the
+FP64 path).  `value` is the chosen form's throughput over all column-steps;
+`converged_value` counts only the column-steps that met the Newton tolerance
+and stayed finite.  The learned form's network is trained first (untimed) on
+columns drawn from other seeds of the same workload: plain steps of all its
+columns, spread evenly over the timed horizon, give the (state, converged
+step) pairs (2,097,152 of them).  The line also carries the 131,072-column
+RichardsModel workload (FP64 and learned) under "rre_131072_uneven".
 """
 import argparse
 import json
@@ -30,22 +35,25 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per step of the step kernels measured with rocprofv3 (FETCH_SIZE
-# and WRITE_SIZE summed over the bins' launches, FETCH scaled x2.0 and WRITE
-# x1.0 by a plain 1 GiB device copy); raw sums beside them.  They include the
-# scratch writes of the register spills (outside the Newton loop) of the
-# mixed-form kernels, which are compiled for four waves per SIMD.  Summary and
-# commands: profiles/k_eh_coop_eh_262144_uneven.md.
-MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1718.7e6, "raw": 1116.7e6},
-                    ("eh_262144_uneven", "mixed"): {"scaled": 1469.1e6, "raw": 985.3e6}}
+# and WRITE_SIZE summed over the bins' launches, FETCH scaled by the factor a
+# plain device copy of known size reads for it, WRITE likewise); raw sums
+# beside them.  Figures, build and commands:
+# profiles/k_eh_coop_eh_262144_uneven.md ("source" says which build).
+MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1718.7e6, "raw": 1116.7e6,
+                                                      "source": "mixed+learned form, "
+                                                                "modified-Picard build"},
+                    ("eh_262144_uneven", "mixed"): {"scaled": 1469.1e6, "raw": 985.3e6,
+                                                    "source": "modified-Picard build"}}
 
 # What bounds the fused step: VALU issue.  rocprofv3 counters of the step
-# kernels on 3 early steps of eh_262144_uneven, mixed form, shapes
-# (16,1),(32,1),(32,1),(64,1) (profiles/k_eh_coop_eh_262144_uneven.md):
+# kernels on 3 early steps of eh_262144_uneven (profiles/k_eh_coop_eh_262144_uneven.md):
 # SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes per step, SQ_INSTS_VALU per step, and
 # the kernels' time per step in the same build's trace.
 MEASURED_VALU = {"eh_262144_uneven": {"fp64_flop_per_step": 4.604e10,
                                       "valu_instructions_per_step": 1.046e9,
-                                      "kernel_ms_per_step": 2.496}}
+                                      "kernel_ms_per_step": 2.496,
+                                      "source": "rocprofv3 counters, 3 early steps, mixed "
+                                                "form, modified-Picard build"}}
 FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 CU_COUNT, ENGINE_GHZ = 256, 2.4
 
@@ -61,20 +69,6 @@ def step_bytes(batch, learned: bool) -> float:
         per = 56 * N + 176 + (16 * N if learned else 0)
     else:
         per = 24 * N + 96 + (8 * N if learned else 0)
-    return float(per.sum())
-
-
-def unfused_bytes(batch, iters: np.ndarray) -> float:
-    """Bytes the unfused hook-by-hook decomposition would move for the same
-    Newton iteration counts ([steps][ncol]): explicit stage plus, per
-    iteration, assemble (aux + residual + Jacobian) and solve, each streaming
-    its operands through HBM (EH 80N+136 + k(280N+96); RRE k(104N+72)+64)."""
-    N = batch.nlayers.astype(np.float64)[None, :]
-    k = (iters & 0xFFFF).astype(np.float64)
-    if batch.model == 1:
-        per = 80 * N + 136 + k * (280 * N + 96)
-    else:
-        per = k * (104 * N + 72) + 64
     return float(per.sum())
 
 
@@ -107,7 +101,7 @@ def parse_shapes(spec):
     return out
 
 
-def train_warm_start(name, ncol, jac, epochs, horizon, collect=8, batch=8192):
+def train_warm_start(name, ncol, jac, epochs, horizon, collect=8, batch=8192, jacobian=0):
     """Collect (state, converged step) pairs with the plain solver on columns
     of other seeds -- `collect` steps spread evenly over the run's `horizon`
     -- and train the warm-start network on the device."""
@@ -116,7 +110,7 @@ def train_warm_start(name, ncol, jac, epochs, horizon, collect=8, batch=8192):
     from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
     bt = inputs.make_batch(inputs.training_workload(name), ncol=ncol, nsteps=horizon)
     w = bt.workload
-    m = model_from_batch(bt, jac_precision=jac, solver=1)
+    m = model_from_batch(bt, jac_precision=jac, solver=1, jacobian=jacobian)
     ws = NewtonWarmStart(m, seed=5)
     sim = LandSimulation(m, bt.copy_state(), w.dt, max_iters=w.newton_max_iters,
                          tol=w.newton_tol)
@@ -140,11 +134,12 @@ def train_warm_start(name, ncol, jac, epochs, horizon, collect=8, batch=8192):
     return params, info
 
 
-def run_form(batch, jac, steps, warmup, solver=0, shapes_spec=None, ws_params=None):
+def run_form(batch, jac, steps, warmup, solver=0, shapes_spec=None, ws_params=None,
+             jacobian=0):
     import torch
     from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
     w = batch.workload
-    m = model_from_batch(batch, jac_precision=jac, solver=solver)
+    m = model_from_batch(batch, jac_precision=jac, solver=solver, jacobian=jacobian)
     if ws_params is not None:
         ws = NewtonWarmStart(m)
         ws.set_params(ws_params)
@@ -201,22 +196,66 @@ def _top_bc(batch):
     return WaterHeatBC(WaterFluxBC(batch.top_water[0]), HeatFluxBC(batch.top_heat[0]))
 
 
+# (jac_precision, jacobian) of each form; "learned" adds the warm start to
+# the form named in LEARNED_ON
+FORMS = {"fp64": (64, 0), "mixed": (32, 0), "newton": (32, 1)}
+LEARNED_ON = {1: "newton", 0: "fp64"}  # by model: EnergyHydrology, RichardsModel
+
+
+def form_summary(batch, steps, r):
+    st = r["stats"]
+    n = batch.ncol * steps
+    return {"value": n / r["wall"], "ms_per_step": r["wall"] * 1e3 / steps,
+            "converged_value": (n - st.n_nonconverged) / r["wall"],
+            "mean_newton_iters": st.mean_iters,
+            "nonconverged_fraction": st.n_nonconverged / n,
+            "nonfinite_fraction": st.n_nan / n,
+            "kernel_ms_per_step": r["kernel_ms"],
+            "warm_start_inference_ms_per_step": r["infer_ms"] or None}
+
+
+def run_forms(batch, steps, warmup, names, args, epochs):
+    """Run the named forms (untimed training first for "learned"); returns
+    ({form: run_form result}, training info)."""
+    w = batch.workload
+    res, ws_info = {}, None
+    for name in names:
+        if name == "learned":
+            jac, jacobian = FORMS[LEARNED_ON[batch.model]]
+            collect = 16 if batch.model == 0 else 8  # 2,097,152 pairs either way
+            params, ws_info = train_warm_start(w.name, batch.ncol, jac, epochs, steps,
+                                               collect=collect, jacobian=jacobian)
+        else:
+            jac, jacobian = FORMS[name]
+            params = None
+        r = run_form(batch, jac, steps, warmup, args.solver, args.shapes, params, jacobian)
+        res[name] = r
+        st = r["stats"]
+        print(f"[bench] {w.name} {name}: {batch.ncol * steps / r['wall']:.4g} column-steps/s, "
+              f"{r['wall'] * 1e3 / steps:.3f} ms/step, mean Newton iters {st.mean_iters:.2f}, "
+              f"non-converged {st.n_nonconverged}, non-finite {st.n_nan}",
+              file=sys.stderr, flush=True)
+    return res, ws_info
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="eh_262144_uneven")
-    ap.add_argument("--form", default="learned", choices=["learned", "mixed", "fp64"])
+    ap.add_argument("--form", default="learned", choices=["learned", "newton", "mixed", "fp64"])
     ap.add_argument("--epochs", type=int, default=16, help="warm-start training epochs")
     ap.add_argument("--ncol", type=int, default=None, help="prefix of the workload (testing)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rre", action="store_true",
+                    help="leave out the rre_131072_uneven key")
     ap.add_argument("--solver", type=int, default=0, help="0 auto, 1 group, 2 lane")
     ap.add_argument("--shapes", default=None,
                     help='fix group shapes instead of autotuning, e.g. "16:16x1,24:32x1"')
     ap.add_argument("--only", action="store_true",
-                    help="run the chosen form only (no plain FP64 comparison)")
+                    help="run the chosen form only (no other forms)")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per step measured with rocprofv3 (FETCH_SIZE, WRITE_SIZE, "
                          "scaled by a copy kernel) to report beside the compulsory bytes")
@@ -236,27 +275,15 @@ def main():
         if steps != w.nsteps else batch
     print(f"[bench] workload {w.name}: {batch.ncol} columns, {batch.ncell} cells, {steps} "
           f"steps", file=sys.stderr, flush=True)
-    forms = {"learned": 32, "mixed": 32, "fp64": 64}
-    if batch.model == 0:  # RichardsModel: the learned form runs on the FP64 Jacobian
-        forms["learned"] = 64
-    chosen = forms[args.form]
-    res = {}
-    ws_params, ws_info = None, None
-    if args.form == "learned":
-        ws_params, ws_info = train_warm_start(w.name, batch.ncol, chosen, args.epochs, steps)
-    for name, jac in (("fp64", 64), ("mixed", 32), ("learned", forms["learned"])):
-        if name != args.form and (args.only or (batch.model == 0 and name == "mixed")):
-            continue
-        if name == "learned" and ws_params is None:
-            continue
-        r = run_form(batch_steps, jac, steps, args.warmup, args.solver, args.shapes,
-                     ws_params if name == "learned" else None)
-        res[name] = r
-        st = r["stats"]
-        print(f"[bench] {name}: {batch.ncol * steps / r['wall']:.4g} column-steps/s, "
-              f"{r['wall'] * 1e3 / steps:.3f} ms/step, mean Newton iters {st.mean_iters:.2f}, "
-              f"non-converged {st.n_nonconverged}, non-finite {st.n_nan}",
-              file=sys.stderr, flush=True)
+    if batch.model == 1:
+        names = ["fp64", "mixed", "newton", "learned"]
+    else:
+        names = ["fp64", "learned"]
+    if args.form not in names:
+        sys.exit(f"bench.py: form {args.form} is not defined for {w.name}")
+    if args.only:
+        names = [args.form]
+    res, ws_info = run_forms(batch_steps, steps, args.warmup, names, args, args.epochs)
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 8
@@ -271,10 +298,12 @@ def main():
     st = r["stats"]
     value = batch.ncol * steps / r["wall"]
     learned = args.form == "learned"
+    jac = FORMS[LEARNED_ON[batch.model] if learned else args.form][0]
     bytes_step = step_bytes(batch_steps, learned)
     achieved = bytes_step / (r["kernel_ms"] * 1e-3) / 1e9
-    unfused = unfused_bytes(batch_steps, r["iters"]) / steps
     kname = "k_eh_coop" if batch.model == 1 else "k_rre_coop"
+    ncs = batch.ncol * steps
+    traffic = MEASURED_TRAFFIC.get((w.name, args.form), {})
     out = {
         "metric": "column-steps/s at Newton residual 1e-8: learned+mixed vs plain FP64 GPU",
         "value": value,
@@ -286,25 +315,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f64" if chosen == 64 else "f64 residual/state, f32 jacobian+solve") +
-                 (", bf16 warm-start network (fp32 accumulate)" if args.form == "learned" else ""),
+        "dtype": ("f64" if jac == 64 else "f64 residual/state, f32 jacobian+solve") +
+                 (", bf16 warm-start network (fp32 accumulate)" if learned else ""),
         "data": "synthetic",
         "config": {"workload": w.name, "columns": batch.ncol, "layers": list(w.layers),
                    "dt_s": w.dt, "newton_tol": w.newton_tol,
                    "newton_max_iters": w.newton_max_iters,
-                   "form": {"learned": "learned warm start + mixed precision",
+                   "form": {"learned": f"learned warm start + {LEARNED_ON[batch.model]} form",
+                            "newton": "mixed precision, exact Newton matrix",
                             "mixed": "mixed precision (FP32 Jacobian + solve)",
                             "fp64": "plain FP64"}[args.form],
                    "solver": "group" if r.get("shapes") else "lane",
                    "shapes": r.get("shapes")},
-        "plain_fp64_value": (batch.ncol * steps / res["fp64"]["wall"]) if "fp64" in res
-        else None,
-        "speedup_vs_plain_fp64": (value / (batch.ncol * steps / res["fp64"]["wall"]))
+        "converged_value": (ncs - st.n_nonconverged) / r["wall"],
+        "plain_fp64_value": (ncs / res["fp64"]["wall"]) if "fp64" in res else None,
+        "speedup_vs_plain_fp64": (value / (ncs / res["fp64"]["wall"]))
         if "fp64" in res else None,
-        "other_forms": {k: {"value": batch.ncol * steps / v["wall"],
-                            "ms_per_step": v["wall"] * 1e3 / steps,
-                            "mean_newton_iters": v["stats"].mean_iters}
-                        for k, v in res.items()},
+        "other_forms": {k: form_summary(batch_steps, steps, v) for k, v in res.items()},
         "warm_start_training": ws_info,
         "warm_start_inference": None if not learned else {
             "ms_per_step": r["infer_ms"],
@@ -313,35 +340,52 @@ def main():
         "newton": {"mean_iters": st.mean_iters, "max_iters": st.max_iters_used,
                    "nonconverged_column_steps": st.n_nonconverged,
                    "nonfinite_column_steps": st.n_nan,
-                   "converged_fraction": 1.0 - (st.n_nonconverged / (batch.ncol * steps))},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                   "nonconverged_fraction": st.n_nonconverged / ncs,
+                   "nonfinite_fraction": st.n_nan / ncs,
+                   "note": "non-converged includes non-finite; a non-finite column stops "
+                           "iterating in the step it turns non-finite and in later steps"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": args.traffic if args.traffic is not None else
-                     MEASURED_TRAFFIC.get((w.name, args.form), {}).get("scaled"),
-                     "traffic_raw": MEASURED_TRAFFIC.get((w.name, args.form), {}).get("raw"),
+                     traffic.get("scaled"),
+                     "traffic_raw": traffic.get("raw"),
+                     "traffic_source": traffic.get("source"),
                      "kernel": kname + " (one launch per depth bin; per step)",
                      "kernel_ms_per_step": r["kernel_ms"],
                      "bytes_per_step": bytes_step,
-                     "note": "fused step: compulsory bytes only; the kernel is bound by FP64 "
-                             "VALU issue, not HBM",
-                     "unfused_equivalent_GBps": unfused / (r["kernel_ms"] * 1e-3) / 1e9,
-                     "unfused_equivalent_frac": unfused / (r["kernel_ms"] * 1e-3) / 1e9
-                     / HBM_PEAK_GBS},
+                     "note": "fused step: achieved = compulsory HBM bytes / kernel time; the "
+                             "kernel is bound by FP64 VALU issue and latency, not HBM"},
     }
     v = MEASURED_VALU.get(w.name)
     if v is not None:
         t = v["kernel_ms_per_step"] * 1e-3
         out["valu_roofline"] = {
-            "kernel": kname, "source": "rocprofv3 counters, 3 early steps, mixed form",
+            "kernel": kname, "source": v["source"],
             "fp64_tflops": v["fp64_flop_per_step"] / t / 1e12,
             "fp64_peak_tflops": FP64_VALU_PEAK_TFLOPS,
             "fp64_frac": v["fp64_flop_per_step"] / t / 1e12 / FP64_VALU_PEAK_TFLOPS,
             "valu_issue_per_cu_cycle": v["valu_instructions_per_step"] /
             (CU_COUNT * t * ENGINE_GHZ * 1e9),
             "valu_issue_peak_per_cu_cycle": 1.0}
+    if batch.model == 1 and not args.no_rre and args.ncol is None:
+        rb = inputs.make_batch("rre_131072_uneven")
+        rsteps = rb.workload.nsteps
+        rres, rinfo = run_forms(rb, rsteps, args.warmup, ["fp64", "learned"], args, args.epochs)
+        rbytes = step_bytes(rb, False)
+        out["rre_131072_uneven"] = {
+            "steps": rsteps,
+            "forms": {k: form_summary(rb, rsteps, v) for k, v in rres.items()},
+            "roofline_fp64": {"bound": "valu", "kernel": "k_rre_coop",
+                              "achieved": rbytes / (rres["fp64"]["kernel_ms"] * 1e-3) / 1e9,
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": rbytes / (rres["fp64"]["kernel_ms"] * 1e-3) / 1e9
+                              / HBM_PEAK_GBS, "bytes_per_step": rbytes},
+            "warm_start_training": rinfo}
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        sample = args.cpu_sample or min(batch.ncol, 24576 if batch.model == 1 else 16384)
+        # about 12 s of CPU work at the oracle's ~5e5 column-steps/s on 16 threads
+        default = max(256, min(batch.ncol, (int(5.9e6 / steps) // 256) * 256))
+        sample = args.cpu_sample or default
         cps, el = cpu_baseline(batch_steps, steps, sample, threads)
         # one core on a sixteenth of the sample, beside the reference's own
         # single-process CPU figures
@@ -350,7 +394,8 @@ def main():
         out["cpu_baseline"] = {"value": cps, "unit": "column-steps/s", "cores": threads,
                                "kind": "port",
                                "sample": f"first {sample} columns x {steps} steps of {w.name}, "
-                                         f"FP64 oracle, OpenMP over columns ({el:.1f} s)",
+                                         f"FP64 oracle (reference Jacobian), OpenMP over "
+                                         f"columns ({el:.1f} s)",
                                "single_core": {"value": cps1, "cores": 1,
                                                "sample": f"first {sample1} columns x {steps} "
                                                          f"steps, one thread ({el1:.1f} s)"}}

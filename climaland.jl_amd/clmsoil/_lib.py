@@ -19,6 +19,7 @@ MODEL_RICHARDS, MODEL_ENERGY_HYDROLOGY = 0, 1
 BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE, BC_TOPMODEL, BC_SURFACE_RUNOFF = 0, 1, 2, 3, 4
 BC_ATMOS, BC_ATMOS_TOPMODEL = 5, 6
 SOLVER_AUTO, SOLVER_GROUP, SOLVER_LANE = 0, 1, 2
+JACOBIAN_PICARD, JACOBIAN_NEWTON = 0, 1
 (P_NU, P_THETA_R, P_VG_ALPHA, P_VG_N, P_KSAT, P_SS, P_NU_SS_OM, P_NU_SS_QUARTZ,
  P_NU_SS_GRAVEL, P_KAPPA_DRY, P_KAPPA_SAT_FROZEN, P_KAPPA_SAT_UNFROZEN, P_RHOC_DS) = range(13)
 NPARAM_RRE, NPARAM_EH = 6, 13
@@ -37,7 +38,7 @@ EXPORTED = (
     "clm_ws_init", "clm_ws_dims", "clm_ws_get_params", "clm_ws_set_params", "clm_ws_enable",
     "clm_ws_collect", "clm_ws_train", "clm_ws_features", "clm_ws_forward", "clm_ws_gradient",
     "clm_ws_time", "clm_ws_guess", "clm_save_checkpoint", "clm_read_checkpoint",
-    "clm_set_surface_fluxes", "clm_set_surface_flux_table",
+    "clm_set_surface_fluxes", "clm_set_surface_flux_table", "clm_soil_conductance",
 )
 
 _dp = C.POINTER(C.c_double)
@@ -51,6 +52,7 @@ class Config(C.Structure):
         ("col_ptr", _ip), ("z_face", _dp), ("params", _dp), ("nparam", C.c_int32),
         ("jac_precision", C.c_int32), ("newton_max_iters", C.c_int32),
         ("newton_tol", C.c_double), ("solver", C.c_int32), ("bc_top", C.c_int32),
+        ("jacobian", C.c_int32), ("reserved_", C.c_int32),
     ]
 
 
@@ -71,7 +73,7 @@ class SurfaceFluxes(C.Structure):
 
 
 class StepStats(C.Structure):
-    _fields_ = [("max_iters_used", C.c_int32), ("pad_", C.c_int32), ("mean_iters", C.c_double),
+    _fields_ = [("max_iters_used", C.c_int32), ("n_stalled", C.c_int32), ("mean_iters", C.c_double),
                 ("n_nonconverged", C.c_int64), ("n_nan", C.c_int64),
                 ("n_domain_error", C.c_int64), ("ms", C.c_double)]
 
@@ -118,6 +120,8 @@ def load(path: str | None = None) -> C.CDLL:
         "clm_run": ([vp, C.c_double, C.c_double, C.c_int32, C.c_int32, C.POINTER(StepStats),
                      _ip], C.c_int),
         "clm_totals": ([vp, _dp, _dp], C.c_int),
+        "clm_soil_conductance": ([vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp],
+                                 C.c_int),
         "clm_last_kernel_ms": ([vp, _dp, _ip], C.c_int),
         "clm_get_shapes": ([vp, _ip, C.c_int32, _ip], C.c_int),
         "clm_set_shape": ([vp, C.c_int32, C.c_int32, C.c_int32], C.c_int),

@@ -196,7 +196,8 @@ class _SoilModel:
 
     def __init__(self, *, parameters, domain: ColumnBatchDomain, boundary_conditions: dict,
                  sources: tuple = (), jac_precision: int = 64, newton_max_iters: int = 3,
-                 newton_tol: float = 0.0, solver: int = L.SOLVER_AUTO):
+                 newton_tol: float = 0.0, solver: int = L.SOLVER_AUTO,
+                 jacobian: int = L.JACOBIAN_PICARD):
         self.parameters = parameters
         self.domain = domain
         self.boundary_conditions = boundary_conditions
@@ -220,9 +221,10 @@ class _SoilModel:
         cfg = L.Config(self.model_id, self._bottom_kind(), domain.ncol,
                        L.iptr(self._col_ptr), L.dptr(self._z_face), L.dptr(self._params),
                        self._params.shape[0], jac_precision, newton_max_iters, newton_tol,
-                       solver, self._top_kind())
+                       solver, self._top_kind(), jacobian, 0)
         L.check(self._lib, None, self._lib.clm_create(C.byref(self._ctx), C.byref(cfg), None))
         self.jac_precision = jac_precision
+        self.jacobian = jacobian
         self.set_forcing()
 
     # -- helpers
@@ -427,6 +429,49 @@ def make_compute_imp_tendency(model: _SoilModel):
     return compute_imp_tendency
 
 
+def total_liq_water_vol_per_area(model: _SoilModel, Y: dict) -> np.ndarray:
+    """ClimaLand.total_liq_water_vol_per_area!(surface_field, model, Y, p, t)
+    (rre.jl:502-511, energy_hydrology.jl:1282-1304): Σ ϑ_l Δz per column,
+    plus θ_i ρ_i/ρ_l Δz for EnergyHydrology, evaluated on the device for the
+    state Y (which becomes the model's device state)."""
+    return _totals(model, Y)[0]
+
+
+def total_energy_per_area(model: _SoilModel, Y: dict) -> np.ndarray:
+    """ClimaLand.total_energy_per_area!(surface_field, model, Y, p, t)
+    (energy_hydrology.jl:1306-1320): Σ ρe_int Δz per column (EnergyHydrology)."""
+    if not model.is_eh:
+        raise TypeError("total_energy_per_area is defined for EnergyHydrology")
+    return _totals(model, Y)[1]
+
+
+def soil_conductance(model: _SoilModel, Y: dict, d_ds: float, evap_p: float,
+                     evap_alpha: float, D_vapor: float):
+    """The surface soil conductance to water vapour that
+    update_soil_evaporation's closure computes (energy_hydrology.jl:1103-1131):
+    soil_conductance(S_l_sfc, S_c, d_ds, evap_p, evap_α, _D_vapor) with
+    S_l_sfc the effective saturation of θ_l extrapolated to the surface
+    (:1322-1368), per column, on the device for the state Y.  Returns
+    (g_soil in m/s, dry soil layer thickness in m)."""
+    if not model.is_eh:
+        raise TypeError("soil_conductance is defined for EnergyHydrology")
+    model._check(model._lib.clm_set_state(model._ctx, C.byref(model._state(Y))))
+    n = model.domain.ncol
+    g, dsl = np.zeros(n), np.zeros(n)
+    model._check(model._lib.clm_soil_conductance(model._ctx, d_ds, evap_p, evap_alpha, D_vapor,
+                                                 L.dptr(g), L.dptr(dsl)))
+    return g, dsl
+
+
+def _totals(model: _SoilModel, Y: dict):
+    model._check(model._lib.clm_set_state(model._ctx, C.byref(model._state(Y))))
+    n = model.domain.ncol
+    w = np.zeros(n)
+    e = np.zeros(n) if model.is_eh else None
+    model._check(model._lib.clm_totals(model._ctx, L.dptr(w), L.dptr(e)))
+    return w, e
+
+
 class HIPJacobian:
     """jac_prototype: the block matrix lives on the device; ldiv solves with it."""
 
@@ -585,6 +630,17 @@ class LandSimulation:
         self._pull()
         return self.Y
 
+    def totals(self):
+        """(water, energy) column totals of the current state, computed on the
+        device (energy is None for RichardsModel)."""
+        m = self.model
+        self._push()
+        n = m.domain.ncol
+        w = np.zeros(n)
+        e = np.zeros(n) if m.is_eh else None
+        m._check(m._lib.clm_totals(m._ctx, L.dptr(w), L.dptr(e)))
+        return w, e
+
     def step(self) -> L.StepStats:
         m = self.model
         self._push()
@@ -713,8 +769,10 @@ def _column_norm(m: _SoilModel, dx: dict, x: dict) -> np.ndarray:
 
 
 def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | None = None,
-                     tol: float | None = None, slot: int = 0, solver: int = L.SOLVER_AUTO):
-    """Build the model a synthetic ColumnBatch describes."""
+                     tol: float | None = None, slot: int = 0, solver: int = L.SOLVER_AUTO,
+                     jacobian: int = L.JACOBIAN_PICARD):
+    """Build the model a synthetic ColumnBatch describes.  jacobian: the
+    Newton matrix of the EnergyHydrology group step (L.JACOBIAN_*)."""
     w = batch.workload
     P = batch.params
     hcm = vanGenuchten(P[L.P_VG_ALPHA], P[L.P_VG_N])
@@ -722,6 +780,8 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
     kw = dict(jac_precision=jac_precision or w.jac_precision,
               newton_max_iters=max_iters or w.newton_max_iters,
               newton_tol=w.newton_tol if tol is None else tol, solver=solver)
+    if jacobian != L.JACOBIAN_PICARD:
+        kw["jacobian"] = jacobian
     if batch.model == L.MODEL_RICHARDS:
         par = RichardsParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R])
         tw = batch.top_water[slot]

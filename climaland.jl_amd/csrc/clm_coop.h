@@ -108,6 +108,20 @@ __device__ __forceinline__ JT recip(JT x) {
         return (JT)1 / x;
 }
 
+// Stall test.  The stopping rule is the reference's, max|Δ| < tol on the
+// increment (MaximumError); where an iterate reaches the region in which
+// effective_saturation clips ϑ_l (soil_hydrology_parameterizations.jl:45-50),
+// ψ stops changing with ϑ_l while dψ/dϑ is huge, the steps shrink below tol
+// and the iteration stops far from a root.  The residual of the last
+// iteration tells the two apart: on 36,157 increment-converged RichardsModel
+// and 23,498 EnergyHydrology column-steps of the oracle it stayed below 1e-4
+// (in ϑ_l, and in ρe_int/ρc_s, kelvin) except where it was above 1e-2, most
+// of those above 1e6.  A column-step that stops with a last residual above
+// kStall is flagged (0x100000) and counted as not converged; a warm-started
+// or Newton-matrix attempt that stalls is repeated from U0 with the
+// reference's iteration, as one that turns non-finite is.
+constexpr double kStall = 1e-3;
+
 // Neighbour exchange of the tridiagonal solves through LDS.  Each group of
 // G lanes owns 2G consecutive entries of its wave's LDS slice: its lanes at
 // [G/2, 3G/2) and identity rows (a = c = d = 0, b = 1/b = 1) in the G/2
@@ -440,7 +454,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     }
     double top_used = top;
     int kit = 0, kit_total = 0;
-    bool conv = false, fell_back = false;
+    bool conv = false, fell_back = false, stalled = false;
     double err = 0.0;
     for (int attempt = 0; attempt < 2; ++attempt) {
     if (attempt == 0 && warm) {  // x0 = U0 + guess; cache_imp!(x0) happens at the loop top
@@ -448,12 +462,13 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         for (int k = 0; k < R; ++k)
             if (on[k]) x[k] = u0[k] + gv[base + gl * R + k];
     }
-    if (attempt == 1) {  // the warm start went non-finite: plain Newton from U0
-        if (!(warm && !isfinite(err))) break;
+    if (attempt == 1) {  // the warm start went non-finite or stalled: plain Newton from U0
+        if (!(warm && (!isfinite(err) || stalled))) break;
         fell_back = true;
 #pragma unroll
         for (int k = 0; k < R; ++k) x[k] = u0[k];
     }
+    bool stl = false;  // this lane's rows of the last residual above kStall
     for (kit = 1; kit <= max_iters; ++kit) {
         double K[R], h[R], iD[R];
 #pragma unroll
@@ -480,6 +495,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
             F[k] = (r == 0) ? bot : -ww * (h[k] - hm);
         }
         JT a[R], b[R], c[R], d[R], y[R];
+        stl = false;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             int r = gl * R + k;
@@ -498,6 +514,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
             // Δz_c R = Δz_c (U0 - x) - Δt (F_up - F_down) - Δt Δz_c sink
             double rhs = dzc[k] * (u0[k] - x[k]) - dt * (Fu - F[k]);
             if (topm) rhs -= dt * (dzc[k] * sink[k]);
+            stl |= on[k] && fabs(rhs) > kStall * dzc[k];
             a[k] = (JT)(dt * w[k]);
             c[k] = (JT)(dt * wu);
             b[k] = (JT)(-dt * (w[k] + wu + wbc) - dzc[k] * iD[k]);
@@ -523,6 +540,10 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     }
     if (kit > max_iters) kit = max_iters;
     kit_total += kit;
+    {
+        const bool any_stl = g_umax<G>((uint32_t)stl) != 0u;  // every lane of the group
+        stalled = conv && any_stl;
+    }
     }
 #pragma unroll
     for (int k = 0; k < R; ++k)
@@ -531,9 +552,10 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     if (gl == 0) {
         iw[col] = iw[col] + dt * (-(top_used - bot) - rss);  // ∫F: -I block, fluxes of the last iteration
         int32_t flag = kit_total;
-        if (tol > 0.0 && !conv) flag |= 0x10000;
+        if (tol > 0.0 && (!conv || stalled)) flag |= 0x10000;
         if (!isfinite(err)) flag |= 0x20000;
         if (fell_back) flag |= 0x80000;
+        if (stalled) flag |= 0x100000;
         iters[col] = flag;
     }
 }
@@ -553,7 +575,20 @@ __global__ __launch_bounds__(256) void k_rre_coop(
 template <typename JT, int R>
 constexpr int kEhMinBlocks = (std::is_same<JT, float>::value && R == 1) ? 4 : 1;
 
-template <int G, int R, typename JT, bool ATM = false>
+// NEWTON: the Newton form of the inexact Newton (no counterpart in the
+// reference).  The same FP64 residual, the same water block (K is lagged, so
+// dψ/dϑ is its whole derivative), and the exact derivative of the energy
+// residual in place of the modified-Picard blocks: with ΔT = Δρe/ρc_s(x) +
+// ∂T/∂ϑ Δϑ (ρc_s and ∂T/∂ϑ = -(T - T_0) ρ_l c_l/ρc_s at θ_l = min(ν - θ_i,
+// ϑ_l) of the iterate) the heat flux of the face below a row changes by
+//   ΔG = cTlo ΔT_below + cThi ΔT + wE (Δh_below - Δh),
+//   cTlo = κ_f/Δz_f - ½ ρ_l c_l K_below gh,  cThi = -κ_f/Δz_f - ½ ρ_l c_l K gh,
+// (gh the face's head gradient).  The energy block then carries the
+// advection of heat by the Darcy flux, is no longer symmetric and changes
+// with every iterate, so it is solved in full each iteration; W_ρϑ has no
+// "- I".  A column whose Newton iterate turns non-finite is stepped again from
+// U0 with the reference's matrices (flag bit 0x80000).
+template <int G, int R, typename JT, bool ATM = false, bool NEWTON = false>
 __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
@@ -588,6 +623,8 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
 
     double xv[R], xi[R], xe[R], u0v[R], u0e[R], zc[R], dzc[R], dzf[R];
     double Kl[R], kl[R], rcs[R], Tp[R], Pp[R], iD[R];
+    // Newton form: ρc_s and ∂T/∂ϑ_l at the iterate
+    double rcx[NEWTON ? R : 1], dTdv[NEWTON ? R : 1];
     Span sp[R];  // saturation span of each row (θ_i is fixed for the step after stage 1)
     bool on[R];
     int bad = 0;
@@ -638,6 +675,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         Pp[k] = a.psi;
         sp[k] = make_span(e.v, e.nu - xi[k]);
         iD[k] = psi_invd(e.v, sp[k], xv[k]).invD;  // Wfact at x = U0
+        if constexpr (NEWTON) newton_dT(e, xv[k], xi[k], Tp[k], rcx[k], dTdv[k]);
     }
     if constexpr (ATM) {
         // update_infiltration_water_flux! (Runoff.jl:234-290) and the top fluxes
@@ -722,7 +760,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     // one or two rows per lane it is reduced once here
     using EFac = typename std::conditional<R == 1, Pcr1Factor<G, JT>, Pcr2Factor<G, JT>>::type;
     EFac efac;
-    if constexpr (R <= 2) {
+    if constexpr (R <= 2 && !NEWTON) {
         JT ea[R], eb[R], ec[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -742,9 +780,11 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     }
     const bool warm = gv != nullptr;
     int kit = 0, kit_total = 0;
-    bool conv = false, fell_back = false;
+    bool conv = false, fell_back = false, stalled = false;
     double err = 0.0;
     for (int attempt = 0; attempt < 2; ++attempt) {
+    // the Newton matrices on the first attempt, the reference's on a fallback
+    const bool nw = NEWTON && attempt == 0;
     if (attempt == 0 && warm) {  // x0 = U0 + guess, cache_imp!(x0) and Wfact(x0)
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -753,14 +793,18 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
                 xv[k] = u0v[k] + gv[base + r];
                 xe[k] = u0e[k] + ge[base + r];
             }
-            Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
+            if constexpr (NEWTON)
+                newton_T(e, xv[k], xi[k], xe[k], Tp[k], rcx[k], dTdv[k]);
+            else
+                Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
             PsiI pd = psi_invd(e.v, sp[k], xv[k]);
             Pp[k] = pd.psi;
             iD[k] = pd.invD;
         }
     }
-    if (attempt == 1) {  // the warm start went non-finite: plain Newton from U0
-        if (!(warm && !isfinite(err))) break;
+    if (attempt == 1) {  // the warm start or the Newton matrices went non-finite or
+                         // stalled: Newton with the reference's matrices from U0
+        if (!((warm || NEWTON) && (!isfinite(err) || stalled))) break;
         fell_back = true;
 #pragma unroll
         for (int k = 0; k < R; ++k) {  // the cache of u_n, re-read (HBM holds u_n until the end)
@@ -779,6 +823,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             iD[k] = psi_invd(e.v, sp[k], xv[k]).invD;
         }
     }
+    bool stl = false;  // this lane's rows of the last residual above kStall
     for (kit = 1; kit <= max_iters; ++kit) {
         double h[R], E[R];
 #pragma unroll
@@ -788,6 +833,9 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         }
         // face below each row: Darcy flux F, heat flux G, advective conductance wE
         double F[R], Gf[R], wE[R];
+        // Newton form: ∂G/∂T of the face below each row (zero on the boundary
+        // faces and below rows past the column's top)
+        double ctl[NEWTON ? R : 1], cth[NEWTON ? R : 1];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             int r = gl * R + k;
@@ -797,9 +845,16 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             wE[k] = (r == 0) ? 0.0 : eK * idzf[k];
             F[k] = (r == 0) ? botw : -Kf[k] * gh;
             Gf[k] = (r == 0) ? both : -kf[k] * ((Tp[k] - Tm) * idzf[k]) - eK * gh;
+            if constexpr (NEWTON) {
+                const double Km = 2.0 * Kf[k] - Kl[k];  // lagged K of the row below
+                const bool face = nw && r > 0 && on[k];
+                ctl[k] = face ? wk[k] - 0.5 * RCP_L * Km * gh : 0.0;
+                cth[k] = face ? -wk[k] - 0.5 * RCP_L * Kl[k] * gh : 0.0;
+            }
         }
         JT a[R], b[R], c[R], d[R], y[R];
         double dRe[R];  // Δz_c R_ρe
+        stl = false;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             int r = gl * R + k;
@@ -819,6 +874,8 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
                 rv -= dt * (dzc[k] * snk_v[k]);
                 dRe[k] -= dt * (dzc[k] * snk_e[k]);
             }
+            stl |= on[k] && (fabs(rv) > kStall * dzc[k] ||
+                             fabs(dRe[k]) > kStall * (dzc[k] * rcs[k]));
             a[k] = (JT)(dt * wK[k]);
             c[k] = (JT)(dt * wu);
             b[k] = (JT)(-dt * (wK[k] + wu) - dzc[k] * iD[k]);
@@ -840,9 +897,36 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             dv[k] = on[k] ? yv[k] * iD[k] : 0.0;
         }
         // y of the row below the lane's first row and above its last (0 past
-        // the column's ends, where the conductances are 0)
-        xc.put(on[R - 1] ? y[R - 1] : (JT)0, on[0] ? y[0] : (JT)0, (JT)0, (JT)0);
-        const double yb = (double)xc.at(-1).x, ya = (double)xc.at(1).y;
+        // the column's ends, where the conductances are 0); in the Newton form
+        // also the temperature change ∂T/∂ϑ Δϑ of those rows
+        double tau[NEWTON ? R : 1];
+        if constexpr (NEWTON) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) tau[k] = nw ? dTdv[k] * dv[k] : 0.0;
+            xc.put(on[R - 1] ? y[R - 1] : (JT)0, on[0] ? y[0] : (JT)0, (JT)tau[R - 1],
+                   (JT)tau[0]);
+        } else {
+            xc.put(on[R - 1] ? y[R - 1] : (JT)0, on[0] ? y[0] : (JT)0, (JT)0, (JT)0);
+        }
+        const Q4<JT> nb = xc.at(-1), na = xc.at(1);
+        const double yb = (double)nb.x, ya = (double)na.y;
+        // Newton form: the known part of each face-below heat-flux change,
+        // cTlo τ_below + cThi τ, and the face-above values from the next row
+        double kt[NEWTON ? R : 1], ctl_u[NEWTON ? R : 1], cth_u[NEWTON ? R : 1],
+            kt_u[NEWTON ? R : 1];
+        if constexpr (NEWTON) {
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                kt[k] = ctl[k] * (k > 0 ? tau[k - 1] : (double)nb.z) + cth[k] * tau[k];
+            xc.put((JT)ctl[0], (JT)cth[0], (JT)kt[0], (JT)0);
+            const Q4<JT> nu = xc.at(1);
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                ctl_u[k] = (k + 1 < R) ? ctl[k + 1] : (double)nu.x;
+                cth_u[k] = (k + 1 < R) ? cth[k + 1] : (double)nu.y;
+                kt_u[k] = (k + 1 < R) ? kt[k + 1] : (double)nu.z;
+            }
+        }
         double e2 = 0.0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
@@ -853,11 +937,19 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             double yp = (k + 1 < R) ? yv[k + 1] : ya;
             double wEu = last ? 0.0 : wEn;
             double MEy = wE[k] * (ym - yv[k]) + wEu * (yp - yv[k]);
-            double rhs = dRe[k] - (dt * MEy - dzc[k] * dv[k]);
-            a[k] = (JT)(dt * wk[k]);
-            c[k] = (JT)(dt * wka[k]);
-            b[k] = (JT)(-dt * (wk[k] + wka[k]) - dzc[k] * rcs[k]);
-            d[k] = (JT)rhs;
+            if (NEWTON && nw) {
+                double rhs = dRe[k] - dt * MEy + dt * (kt_u[k] - kt[k]);
+                a[k] = (JT)(dt * ctl[k]);
+                c[k] = (JT)(-dt * cth_u[k]);
+                b[k] = (JT)(-dzc[k] * rcx[k] + dt * (cth[k] - ctl_u[k]));
+                d[k] = (JT)rhs;
+            } else {
+                double rhs = dRe[k] - (dt * MEy - dzc[k] * dv[k]);
+                a[k] = (JT)(dt * wk[k]);
+                c[k] = (JT)(dt * wka[k]);
+                b[k] = (JT)(-dt * (wk[k] + wka[k]) - dzc[k] * rcs[k]);
+                d[k] = (JT)rhs;
+            }
             if (!on[k]) {
                 a[k] = 0;
                 b[k] = 1;
@@ -867,20 +959,29 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             xv[k] -= dv[k];
             e2 = nan_max(e2, fabs(dv[k]));
         }
-        if constexpr (R == 1)
+        if constexpr (R == 1 && !NEWTON)
             y[0] = efac.solve(xc, d[0]);  // y = Δρe / ρc_s
-        else if constexpr (R == 2)
+        else if constexpr (R == 2 && !NEWTON)
             efac.solve(xc, d, y);
         else
             pcr_solve<G, R, JT>(xc, a, b, c, d, y);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             double yr = on[k] ? (double)y[k] : 0.0;
-            xe[k] -= yr * rcs[k];
-            e2 = nan_max(e2, fabs(yr));
+            // Newton form: y = Δρe / ρc_s(x); the norm is Δρe / ρc_s(θ_l lagged)
+            if (NEWTON && nw) {
+                xe[k] -= yr * rcx[k];
+                e2 = nan_max(e2, fabs(yr * (rcx[k] / rcs[k])));
+            } else {
+                xe[k] -= yr * rcs[k];
+                e2 = nan_max(e2, fabs(yr));
+            }
             // cache_imp!(x): T and ψ at the new iterate (energy_hydrology.jl:427-445);
             // dψ/dϑ of the same evaluation is the next Wfact(x)
-            Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
+            if constexpr (NEWTON)
+                newton_T(e, xv[k], xi[k], xe[k], Tp[k], rcx[k], dTdv[k]);
+            else
+                Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
             PsiI pd = psi_invd(e.v, sp[k], xv[k]);
             Pp[k] = pd.psi;
             iD[k] = pd.invD;
@@ -891,6 +992,10 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     }
     if (kit > max_iters) kit = max_iters;
     kit_total += kit;
+    {
+        const bool any_stl = g_umax<G>((uint32_t)stl) != 0u;  // every lane of the group
+        stalled = conv && any_stl;
+    }
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
@@ -908,10 +1013,11 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         iw[col] = iw0 + dt * (-(topw - botw) - rss);
         ie[col] = ie[col] + dt * (-(toph - both) - ress);
         int32_t flag = kit_total;
-        if (tol > 0.0 && !conv) flag |= 0x10000;
+        if (tol > 0.0 && (!conv || stalled)) flag |= 0x10000;
         if (!isfinite(err)) flag |= 0x20000;
         if (bad_g) flag |= 0x40000;
         if (fell_back) flag |= 0x80000;
+        if (stalled) flag |= 0x100000;
         iters[col] = flag;
     }
 }

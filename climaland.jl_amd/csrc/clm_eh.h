@@ -70,6 +70,22 @@ __device__ __forceinline__ double eh_imp_T(const EHPar& e, double vl, double ti,
     return temperature(re, ti, heat_capacity(fmin(e.nu - ti, vl), ti, e.rhoc_ds));
 }
 
+// Newton form (no counterpart in the reference): ρc_s at θ_l = min(ν-θ_i, ϑ_l)
+// of the iterate and ∂T/∂ϑ_l = -(T - T_0) ρ_l c_l / ρc_s where ϑ_l < ν - θ_i
+// (0 above, where θ_l is capped), for a given T ...
+__device__ __forceinline__ void newton_dT(const EHPar& e, double vl, double ti, double T,
+                                          double& rc, double& dTdv) {
+    rc = heat_capacity(fmin(e.nu - ti, vl), ti, e.rhoc_ds);
+    dTdv = vl < e.nu - ti ? -(T - T_0) * RCP_L / rc : 0.0;
+}
+// ... and with T of the implicit cache, eh_imp_T's arithmetic
+__device__ __forceinline__ void newton_T(const EHPar& e, double vl, double ti, double re,
+                                         double& T, double& rc, double& dTdv) {
+    rc = heat_capacity(fmin(e.nu - ti, vl), ti, e.rhoc_ds);
+    T = temperature(re, ti, rc);
+    dTdv = vl < e.nu - ti ? -(T - T_0) * RCP_L / rc : 0.0;
+}
+
 // ---- AtmosDrivenFluxBC, soil-only form (boundary_conditions.jl:880-915) --------
 // The turbulent fluxes and net radiation come from the caller (SurfaceFluxes /
 // radiation drivers are outside this path), per column:

@@ -134,6 +134,26 @@ __host__ __device__ inline VG make_vg(double alpha, double n, double theta_r, do
     return v;
 }
 
+// vanGenuchten's S_c, retention_models.jl:30-44: (1 + ((n-1)/n)^(1-2n))^(-m)
+__host__ __device__ inline double vg_sc(double n) {
+    const double m = 1.0 - 1.0 / n;
+    return pow(1.0 + pow((n - 1.0) / n, 1.0 - 2.0 * n), -m);
+}
+
+// dry_soil_layer_thickness, energy_hydrology.jl:1352-1368 (S_l in (0, 1])
+__device__ __forceinline__ double dry_soil_layer_thickness(double S_l, double aSc, double d_ds,
+                                                           double p) {
+    aSc = fmin(fmax(aSc, 0.0), 1.0);
+    return S_l < aSc ? d_ds * pow((aSc - S_l) / S_l, p) : 0.0;
+}
+
+// soil_conductance, :1322-1350: vapour conductance D_vapor / max(dsl, eps) (m/s)
+__device__ __forceinline__ double soil_conductance(double S_l, double S_c, double d_ds, double p,
+                                                   double alpha, double D_vapor) {
+    const double dsl = dry_soil_layer_thickness(S_l, alpha * S_c, d_ds, p);
+    return D_vapor / fmax(dsl, DEPS);
+}
+
 // effective_saturation, soil_hydrology_parameterizations.jl:45-50
 __device__ __forceinline__ double eff_sat(const VG& v, double nu_eff, double vl) {
     double vs = fmax(vl, v.thr_safe);

@@ -25,7 +25,7 @@ namespace {
 thread_local std::string g_create_error;
 
 struct DevStats {
-    unsigned long long sum_iters, nonconv, nan, domain, nrec;
+    unsigned long long sum_iters, nonconv, nan, domain, nrec, stalled;
     unsigned int max_iters;
     unsigned int pad;
 };
@@ -34,6 +34,7 @@ struct DevStats {
 
 struct clm_ctx {
     int model = 0, bc_bottom = 0, bc_top = 0, jac_prec = 64, max_iters = 3;
+    int jacobian = CLM_JACOBIAN_PICARD;
     double tol = 0.0;
     int64_t ncol = 0, ncell = 0;
     int nparam = 0;
@@ -305,9 +306,93 @@ int download_state(clm_ctx* c, double* const* src, clm_state* Y) {
     return CLM_OK;
 }
 
+// Column totals (total_liq_water_vol_per_area!, rre.jl:502-511 and
+// energy_hydrology.jl:1282-1304; total_energy_per_area!, :1306-1320):
+// column_integral_definite of ϑ_l (+ θ_i ρ_i/ρ_l) and of ρe_int, Σ f Δz over
+// the layers.  One column per lane: CSR order (group path) ...
+__global__ __launch_bounds__(256) void k_totals_csr(const int32_t* __restrict__ col_ptr,
+                                                    const double* __restrict__ zf,
+                                                    const double* __restrict__ vl,
+                                                    const double* __restrict__ ti,
+                                                    const double* __restrict__ re, int64_t ncol,
+                                                    double* __restrict__ water,
+                                                    double* __restrict__ energy) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncol) return;
+    const int64_t a = col_ptr[c], n = col_ptr[c + 1] - a;
+    const double* z = zf + a + c;
+    double w = 0.0, e = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double dz = z[i + 1] - z[i];
+        double vol = vl[a + i];
+        if (ti) {
+            vol += ti[a + i] * RHO_I / RHO_L;
+            e += re[a + i] * dz;
+        }
+        w += vol * dz;
+    }
+    water[c] = w;
+    if (energy) energy[c] = e;
+}
+// ... and the packed lane layout (slot order; a level is one coalesced row)
+__global__ __launch_bounds__(256) void k_totals_packed(Layout L, const double* __restrict__ zf,
+                                                       const double* __restrict__ vl,
+                                                       const double* __restrict__ ti,
+                                                       const double* __restrict__ re,
+                                                       double* __restrict__ water,
+                                                       double* __restrict__ energy) {
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= L.nslot) return;
+    const int64_t cb = L.cell_off[s >> 6] + (s & 63), fb = L.face_off[s >> 6] + (s & 63);
+    const int n = L.nl[s];
+    double w = 0.0, e = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double dz = zf[fb + 64 * (j + 1)] - zf[fb + 64 * j];
+        double vol = vl[cb + 64 * j];
+        if (ti) {
+            vol += ti[cb + 64 * j] * RHO_I / RHO_L;
+            e += re[cb + 64 * j] * dz;
+        }
+        w += vol * dz;
+    }
+    water[s] = w;
+    if (energy) energy[s] = e;
+}
+
+// Surface soil conductance to water vapour (energy_hydrology.jl:1103-1131):
+// θ_l of the top two cells (volumetric_liquid_fraction with ν - θ_i) extrapolated
+// linearly to the surface (linear_interpolation_to_surface!, Domains.jl:883-895),
+// floored at θ_r + eps, its effective saturation with ν, then
+// soil_conductance(S_l, S_c, d_ds, evap_p, evap_α, D_vapor).  One column per
+// lane, CSR state of the group path.
+__global__ __launch_bounds__(256) void k_soil_conductance(
+    const int32_t* __restrict__ col_ptr, const double* __restrict__ zf,
+    const double* __restrict__ par, int64_t ncol, const double* __restrict__ vl,
+    const double* __restrict__ ti, double d_ds, double evap_p, double evap_alpha, double D_vapor,
+    double* __restrict__ g_out, double* __restrict__ dsl_out) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= ncol) return;
+    const int64_t a = col_ptr[c], n = col_ptr[c + 1] - a;
+    const double* z = zf + a + c;
+    const VG v = make_vg(par[P_VG_ALPHA * ncol + c], par[P_VG_N * ncol + c],
+                         par[P_THETA_R * ncol + c], par[P_SS * ncol + c], par[P_KSAT * ncol + c]);
+    const double nu = par[P_NU * ncol + c];
+    const int64_t t1 = a + n - 1, t2 = a + n - 2;
+    const double f1 = liq_fraction(v, vl[t1], nu - ti[t1]);
+    const double f2 = liq_fraction(v, vl[t2], nu - ti[t2]);
+    const double z1 = 0.5 * (z[n - 1] + z[n]), z2 = 0.5 * (z[n - 2] + z[n - 1]);
+    const double dz_top = z[n] - z1;
+    double th = (f1 - f2) / (z1 - z2) * (dz_top + z1 - z2) + f2;
+    th = fmax(th, v.theta_r + DEPS);
+    const double S = eff_sat(v, nu, th);
+    const double Sc = vg_sc(v.n);
+    g_out[c] = soil_conductance(S, Sc, d_ds, evap_p, evap_alpha, D_vapor);
+    if (dsl_out) dsl_out[c] = dry_soil_layer_thickness(S, evap_alpha * Sc, d_ds, evap_p);
+}
+
 __global__ void k_zero_stats(DevStats* s) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        s->sum_iters = s->nonconv = s->nan = s->domain = s->nrec = 0;
+        s->sum_iters = s->nonconv = s->nan = s->domain = s->nrec = s->stalled = 0;
         s->max_iters = 0;
     }
 }
@@ -316,7 +401,7 @@ __global__ __launch_bounds__(256) void k_reduce_stats(const int32_t* __restrict_
                                                       const int32_t* __restrict__ nl,
                                                       int64_t nslot, int64_t nrec,
                                                       DevStats* __restrict__ st) {
-    unsigned long long sum = 0, nc = 0, nn = 0, nd = 0, cnt = 0;
+    unsigned long long sum = 0, nc = 0, nn = 0, nd = 0, cnt = 0, ns = 0;
     unsigned int mx = 0;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrec;
          r += (int64_t)gridDim.x * blockDim.x) {
@@ -328,6 +413,7 @@ __global__ __launch_bounds__(256) void k_reduce_stats(const int32_t* __restrict_
         nc += (v & 0x10000) ? 1 : 0;
         nn += (v & 0x20000) ? 1 : 0;
         nd += (v & 0x40000) ? 1 : 0;
+        ns += (v & 0x100000) ? 1 : 0;
         cnt += 1;
     }
     // wavefront reduction then one atomic per wave
@@ -337,6 +423,7 @@ __global__ __launch_bounds__(256) void k_reduce_stats(const int32_t* __restrict_
         nn += __shfl_down(nn, off);
         nd += __shfl_down(nd, off);
         cnt += __shfl_down(cnt, off);
+        ns += __shfl_down(ns, off);
         unsigned int o = __shfl_down(mx, off);
         mx = o > mx ? o : mx;
     }
@@ -346,23 +433,27 @@ __global__ __launch_bounds__(256) void k_reduce_stats(const int32_t* __restrict_
         atomicAdd(&st->nan, nn);
         atomicAdd(&st->domain, nd);
         atomicAdd(&st->nrec, cnt);
+        atomicAdd(&st->stalled, ns);
         atomicMax(&st->max_iters, mx);
     }
 }
 
-int collect_stats(clm_ctx* c, int64_t nsteps, clm_step_stats* out) {
-    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, c->d_stats);
+int collect_stats(clm_ctx* c, int64_t nsteps, clm_step_stats* out,
+                  const int32_t* iters = nullptr, DevStats* dst = nullptr) {
+    if (!iters) iters = c->d_iters;
+    if (!dst) dst = c->d_stats;
+    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, c->stream, dst);
     int64_t nv = ncolvec(c);
     int64_t nrec = nsteps * nv;
     unsigned g = (unsigned)std::min<int64_t>((nrec + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_reduce_stats, dim3(g), dim3(256), 0, c->stream, c->d_iters,
-                       is_group(c) ? nullptr : c->d_nl, nv, nrec, c->d_stats);
+    hipLaunchKernelGGL(k_reduce_stats, dim3(g), dim3(256), 0, c->stream, iters,
+                       is_group(c) ? nullptr : c->d_nl, nv, nrec, dst);
     HIPCHK(c, hipGetLastError());
     DevStats h;
-    HIPCHK(c, hipMemcpyAsync(&h, c->d_stats, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&h, dst, sizeof h, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     out->max_iters_used = (int32_t)h.max_iters;
-    out->pad_ = 0;
+    out->n_stalled = (int32_t)std::min<unsigned long long>(h.stalled, 0x7fffffffull);
     out->mean_iters = h.nrec ? (double)h.sum_iters / (double)h.nrec : 0.0;
     out->n_nonconverged = (int64_t)h.nonconv;
     out->n_nan = (int64_t)h.nan;
@@ -370,8 +461,14 @@ int collect_stats(clm_ctx* c, int64_t nsteps, clm_step_stats* out) {
     return CLM_OK;
 }
 
+void destroy_graph(clm_ctx* c);
+
 int ensure_iters(clm_ctx* c, int64_t nsteps) {
     if (nsteps <= c->iters_cap) return CLM_OK;
+    // a captured run graph writes the old table: drop it with the table, and
+    // keep iters_cap at 0 until the new one exists
+    destroy_graph(c);
+    c->iters_cap = 0;
     dfree(c->d_iters);
     int r = dalloc(c, &c->d_iters, nsteps * ncolvec(c));
     if (r) return r;
@@ -418,6 +515,8 @@ struct StepExtra {
     const double* ge = nullptr;
     double* u0out = nullptr;
     Surface sf{};  // AtmosDrivenFluxBC inputs per column (EH)
+    // state in other device memory (CSR order) instead of the context's
+    double *xv = nullptr, *xi = nullptr, *xe = nullptr, *iw = nullptr, *ie = nullptr;
 };
 
 bool is_atmos(const clm_ctx* c) {
@@ -447,23 +546,32 @@ void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const dou
                 double dt, int32_t* iters, const StepExtra& x) {
     constexpr int gpb = 256 / G;  // columns per 256-thread block
     dim3 g((unsigned)((bn.n + gpb - 1) / gpb)), b(256);
+    double* xv = x.xv ? x.xv : c->d_cv;
+    double* xi = x.xv ? x.xi : c->d_ci;
+    double* xe = x.xv ? x.xe : c->d_ce;
+    double* iw = x.xv ? x.iw : c->d_ciw;
+    double* ie = x.xv ? x.ie : c->d_cie;
     if (!is_eh(c)) {
         hipLaunchKernelGGL((k_rre_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
-                           bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ciw, ftw, c->d_cfbw,
+                           bn.n, c->d_cpar, c->ncol, c->d_czf, xv, iw, ftw, c->d_cfbw,
                            rre_bc(c), dt, c->max_iters, c->tol, iters, x.gv, x.u0out);
     } else {
         if (is_atmos(c))
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, true>), g, b, 0, c->stream, c->d_col_ptr,
-                               bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci,
-                               c->d_ce, c->d_ciw, c->d_cie, ftw, fth, c->d_cfbw, c->d_cfbh,
-                               c->bc_bottom, dt, c->max_iters, c->tol, iters, x.gv, x.ge,
-                               x.u0out, c->bc_top, x.sf);
+                               bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe, iw, ie,
+                               ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
+                               c->tol, iters, x.gv, x.ge, x.u0out, c->bc_top, x.sf);
+        else if (c->jacobian == CLM_JACOBIAN_NEWTON)
+            hipLaunchKernelGGL((k_eh_coop<G, R, JT, false, true>), g, b, 0, c->stream,
+                               c->d_col_ptr, bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv,
+                               xi, xe, iw, ie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt,
+                               c->max_iters, c->tol, iters, x.gv, x.ge, x.u0out, c->bc_top,
+                               x.sf);
         else
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, false>), g, b, 0, c->stream, c->d_col_ptr,
-                               bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, c->d_cv, c->d_ci,
-                               c->d_ce, c->d_ciw, c->d_cie, ftw, fth, c->d_cfbw, c->d_cfbh,
-                               c->bc_bottom, dt, c->max_iters, c->tol, iters, x.gv, x.ge,
-                               x.u0out, c->bc_top, x.sf);
+                               bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe, iw, ie,
+                               ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
+                               c->tol, iters, x.gv, x.ge, x.u0out, c->bc_top, x.sf);
     }
 }
 
@@ -569,17 +677,22 @@ __global__ __launch_bounds__(256) void k_rg_scatter(int32_t* __restrict__ cols,
     if (i < n) cols[h[key] + rank] = col;
 }
 
-void launch_regroup(clm_ctx* c, const int32_t* iters) {
+// tmp: ncol entries; hist, fill: kNBucket * kRgBuckets entries, hist zero on
+// entry (k_rg_scan leaves it zero again)
+void launch_regroup(clm_ctx* c, const int32_t* iters, int32_t* tmp, int32_t* hist,
+                    int32_t* fill) {
     BinOffsets bo;
     for (int b = 0; b <= kNBucket; ++b) bo.off[b] = c->bin_off[b];
     const int n = (int)c->ncol;
     const dim3 g((unsigned)((n + 255) / 256));
-    hipLaunchKernelGGL(k_rg_count, g, dim3(256), 0, c->stream, c->d_bincols, c->d_bintmp, n, bo,
-                       iters, c->d_rghist);
-    hipLaunchKernelGGL(k_rg_scan, dim3(1), dim3(64 * kNBucket), 0, c->stream, bo, c->d_rghist,
-                       c->d_rgfill);
-    hipLaunchKernelGGL(k_rg_scatter, g, dim3(256), 0, c->stream, c->d_bincols, c->d_bintmp, n, bo,
-                       iters, c->d_rgfill);
+    hipLaunchKernelGGL(k_rg_count, g, dim3(256), 0, c->stream, c->d_bincols, tmp, n, bo, iters,
+                       hist);
+    hipLaunchKernelGGL(k_rg_scan, dim3(1), dim3(64 * kNBucket), 0, c->stream, bo, hist, fill);
+    hipLaunchKernelGGL(k_rg_scatter, g, dim3(256), 0, c->stream, c->d_bincols, tmp, n, bo, iters,
+                       fill);
+}
+void launch_regroup(clm_ctx* c, const int32_t* iters) {
+    launch_regroup(c, iters, c->d_bintmp, c->d_rghist, c->d_rgfill);
 }
 
 void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
@@ -959,6 +1072,13 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
         return set_err(nullptr, CLM_E_ARG,
                        "AtmosDrivenFluxBC is implemented for the group solver (columns of up "
                        "to 128 layers)");
+    if (cfg->jacobian != CLM_JACOBIAN_PICARD && cfg->jacobian != CLM_JACOBIAN_NEWTON)
+        return set_err(nullptr, CLM_E_ARG, "unknown jacobian %d", cfg->jacobian);
+    if (cfg->jacobian == CLM_JACOBIAN_NEWTON &&
+        (cfg->model != CLM_MODEL_ENERGY_HYDROLOGY || solver != CLM_SOLVER_GROUP || atmos))
+        return set_err(nullptr, CLM_E_ARG,
+                       "the Newton matrix is implemented for EnergyHydrology on the group "
+                       "solver with a prescribed-flux top boundary");
 
     clm_ctx* c = new clm_ctx();
     c->solver = solver;
@@ -966,6 +1086,7 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     c->bc_bottom = cfg->bc_bottom;
     c->bc_top = cfg->bc_top;
     c->jac_prec = cfg->jac_precision;
+    c->jacobian = cfg->jacobian;
     c->max_iters = cfg->newton_max_iters;
     c->tol = cfg->newton_tol;
     c->ncol = cfg->ncol;
@@ -1330,13 +1451,17 @@ int clm_set_forcing_table(clm_ctx* c, int32_t nslots, const double* tw, const do
     if (!c) return CLM_E_ARG;
     if (nslots < 1 || !tw) return set_err(c, CLM_E_ARG, "forcing table needs nslots >= 1");
     if (nslots != c->tab_slots) {
+        // the captured run graph points at the old table: drop it, and leave
+        // the context without a table until both allocations succeed (clm_run
+        // then reports CLM_E_STATE instead of replaying freed memory)
+        destroy_graph(c);
+        c->tab_slots = 0;
         dfree(c->d_tab_w);
         dfree(c->d_tab_h);
         int r;
         if ((r = dalloc(c, &c->d_tab_w, (int64_t)nslots * ncolvec(c)))) return r;
         if ((r = dalloc(c, &c->d_tab_h, (int64_t)nslots * ncolvec(c)))) return r;
         c->tab_slots = nslots;
-        destroy_graph(c);
     }
     if (is_group(c)) {
         int r;
@@ -1377,15 +1502,12 @@ int clm_set_surface_flux_table(clm_ctx* c, int32_t nslots, const clm_surface_flu
     if (nslots < 1) return set_err(c, CLM_E_ARG, "surface flux table needs nslots >= 1");
     int r;
     if (nslots != c->sfc_slots) {
-        for (int k = 0; k < 7; ++k) {
-            dfree(c->d_tab_sfc[k]);
-            c->d_tab_sfc[k] = nullptr;
-        }
+        destroy_graph(c);
         c->sfc_slots = 0;
+        for (int k = 0; k < 7; ++k) dfree(c->d_tab_sfc[k]);
         for (int k = 0; k < 7; ++k)
             if ((r = dalloc(c, &c->d_tab_sfc[k], (int64_t)nslots * c->ncol))) return r;
         c->sfc_slots = nslots;
-        destroy_graph(c);
     }
     for (int k = 0; k < 7; ++k)
         if ((r = upload_raw(c, sfc_field(s, k), c->d_tab_sfc[k], (int64_t)nslots * c->ncol)))
@@ -1745,43 +1867,41 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
 
 int clm_totals(clm_ctx* c, double* water, double* energy) {
     if (!c || !water) return CLM_E_ARG;
-    std::vector<double> v((size_t)c->ncell), t, e;
-    clm_state Y = {v.data(), nullptr, nullptr, nullptr, nullptr};
-    std::vector<double> iw((size_t)c->ncol), ie((size_t)c->ncol);
-    Y.intF_water = iw.data();
-    if (is_eh(c)) {
-        t.resize((size_t)c->ncell);
-        e.resize((size_t)c->ncell);
-        Y.theta_i = t.data();
-        Y.rho_e_int = e.data();
-        Y.intF_energy = ie.data();
+    const bool eh = is_eh(c);
+    double* dw = c->d_wd[3];  // per-column work vectors of the fine hooks (ncol or nslot)
+    double* de = eh ? c->d_wd[4] : nullptr;
+    if (is_group(c)) {
+        hipLaunchKernelGGL(k_totals_csr, dim3((unsigned)((c->ncol + 255) / 256)), dim3(256), 0,
+                           c->stream, c->d_col_ptr, c->d_czf, c->d_cv, eh ? c->d_ci : nullptr,
+                           eh ? c->d_ce : nullptr, c->ncol, dw, de);
+        HIPCHK(c, hipGetLastError());
+        int r;
+        if ((r = download_raw(c, dw, water, c->ncol))) return r;
+        return energy && eh ? download_raw(c, de, energy, c->ncol) : CLM_OK;
     }
-    int r = clm_get_state(c, &Y);
-    if (r) return r;
-    // zf in CSR order from the packed copy
-    std::vector<double> zf((size_t)c->npface);
-    HIPCHK(c, hipMemcpyAsync(zf.data(), c->d_zf, 8 * zf.size(), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int64_t s = 0; s < c->nslot; ++s) {
-        int32_t col = c->slot_col[s];
-        if (col < 0) continue;
-        int64_t fb = c->face_off[s >> 6] + (s & 63);
-        double w = 0.0, en = 0.0;
-        for (int32_t j = 0; j < c->slot_nl[s]; ++j) {
-            double dz = zf[fb + 64 * (j + 1)] - zf[fb + 64 * j];
-            int64_t i = c->col_ptr[col] + j;
-            double vol = v[i];
-            if (is_eh(c)) {
-                vol += t[i] * RHO_I / RHO_L;
-                en += e[i] * dz;
-            }
-            w += vol * dz;
-        }
-        water[col] = w;
-        if (energy) energy[col] = en;
-    }
-    return CLM_OK;
+    hipLaunchKernelGGL(k_totals_packed, grid_slots(c), dim3(256), 0, c->stream, layout(c),
+                       c->d_zf, c->d_xv, eh ? c->d_xi : nullptr, eh ? c->d_xe : nullptr, dw, de);
+    HIPCHK(c, hipGetLastError());
+    int r;
+    if ((r = download_cols(c, dw, water))) return r;
+    return energy && eh ? download_cols(c, de, energy) : CLM_OK;
+}
+
+int clm_soil_conductance(clm_ctx* c, double d_ds, double evap_p, double evap_alpha,
+                         double D_vapor, double* g_soil, double* dry_layer) {
+    if (!c || !g_soil) return CLM_E_ARG;
+    if (!is_eh(c) || !is_group(c))
+        return set_err(c, CLM_E_STATE,
+                       "clm_soil_conductance: EnergyHydrology on the group solver only");
+    double* dg = c->d_wd[3];
+    double* dd = dry_layer ? c->d_wd[4] : nullptr;
+    hipLaunchKernelGGL(k_soil_conductance, dim3((unsigned)((c->ncol + 255) / 256)), dim3(256), 0,
+                       c->stream, c->d_col_ptr, c->d_czf, c->d_cpar, c->ncol, c->d_cv, c->d_ci,
+                       d_ds, evap_p, evap_alpha, D_vapor, dg, dd);
+    HIPCHK(c, hipGetLastError());
+    int r;
+    if ((r = download_raw(c, dg, g_soil, c->ncol))) return r;
+    return dd ? download_raw(c, dd, dry_layer, c->ncol) : CLM_OK;
 }
 
 int clm_get_shapes(clm_ctx* c, int32_t* rec, int32_t max_records, int32_t* nrec) {

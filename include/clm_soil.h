@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define CLM_ABI_VERSION 2
+#define CLM_ABI_VERSION 3
 
 enum {
     CLM_OK = 0,
@@ -130,6 +130,18 @@ enum {
     CLM_SOLVER_LANE = 2   /* one column per lane, Thomas through HBM scratch */
 };
 
+/* The Newton matrix of the EnergyHydrology group step (clm_step / clm_run).
+ * PICARD: the reference's modified-Picard Jacobian (energy_hydrology.jl:
+ * 466-576) -- the fine hooks always use it.  NEWTON: the exact derivative of
+ * the same residual (no counterpart in the reference): the water block is the
+ * reference's; the energy block adds the advection of heat by the Darcy flux
+ * (∂(ρe_l K)/∂T) and takes ρc_s at the iterate's θ_l, and W_ρϑ carries ∂T/∂ϑ_l
+ * and no "- I".  Same residual, same stopping rule, same root: an inexact
+ * Newton that converges in fewer iterations.  A column whose Newton iterate
+ * turns non-finite repeats the step with the reference's matrices (flag bit
+ * 0x80000).  EnergyHydrology, group solver, prescribed-flux top boundary. */
+enum { CLM_JACOBIAN_PICARD = 0, CLM_JACOBIAN_NEWTON = 1 };
+
 typedef struct clm_ctx clm_ctx;
 
 typedef struct {
@@ -149,6 +161,8 @@ typedef struct {
     int32_t solver;           /* CLM_SOLVER_*: how clm_step / clm_run solve */
     int32_t bc_top;           /* CLM_BC_FLUX, CLM_BC_MOISTURE_STATE, CLM_BC_TOPMODEL or
                                  CLM_BC_SURFACE_RUNOFF */
+    int32_t jacobian;         /* CLM_JACOBIAN_* (0: the reference's) */
+    int32_t reserved_;
 } clm_config;
 
 /* Prognostic state Y.soil.  RRE: vartheta_l, intF_water.
@@ -199,9 +213,12 @@ typedef struct {
 
 typedef struct {
     int32_t max_iters_used;   /* max Newton iterations over columns */
-    int32_t pad_;
+    int32_t n_stalled;        /* column-steps whose increments met tol while the last
+                                 residual stayed above 1e-3 (in ϑ_l, and in ρe_int/ρc_s
+                                 in kelvin): stopped away from a root; counted in
+                                 n_nonconverged too (saturates at INT32_MAX) */
     double mean_iters;        /* mean Newton iterations per column-step */
-    int64_t n_nonconverged;   /* column-steps that hit max_iters with tol > 0 */
+    int64_t n_nonconverged;   /* column-steps that hit max_iters with tol > 0, or stalled */
     int64_t n_nan;            /* columns with a non-finite state after the step */
     int64_t n_domain_error;   /* phase change reached inverse_matric_potential(ψ>0) */
     double ms;                /* device time of the step(s), HIP events */
@@ -251,7 +268,13 @@ int clm_get_jacobian(clm_ctx* ctx, double* W_ww, double* W_ee, double* W_ew);
 int clm_step(clm_ctx* ctx, double t, double dt, clm_step_stats* stats);
 /* nsteps steps with the device forcing table; the whole run is one graph
  * replay after the first call with the same (nsteps, steps_per_slot).
- * iters_out: NULL or host [nsteps][ncol] Newton iteration counts. */
+ * iters_out: NULL or host [nsteps][ncol] Newton iteration counts in the low
+ * 16 bits, with flags: 0x10000 not converged (max_iters reached, or stalled),
+ * 0x20000 non-finite iterate, 0x40000 inverse_matric_potential domain error,
+ * 0x80000 the step was repeated from U0 with the reference's iteration (a
+ * warm-started or Newton-matrix attempt went non-finite or stalled; the count
+ * includes both attempts), 0x100000 stalled: max|Δ| < tol with the last
+ * residual above 1e-3 (in ϑ_l, and in ρe_int/ρc_s in kelvin). */
 int clm_run(clm_ctx* ctx, double t0, double dt, int32_t nsteps, int32_t steps_per_slot,
             clm_step_stats* stats, int32_t* iters_out);
 
@@ -259,6 +282,18 @@ int clm_run(clm_ctx* ctx, double t0, double dt, int32_t nsteps, int32_t steps_pe
  * device state (total_liq_water_vol_per_area!, rre.jl:502-511,
  * energy_hydrology.jl:1282-1320).  energy may be NULL. */
 int clm_totals(clm_ctx* ctx, double* water, double* energy);
+
+/* Surface soil conductance to water vapour of the current device state, per
+ * column (EnergyHydrology, group solver): the θ_l of the top two cells
+ * extrapolated to the surface (linear_interpolation_to_surface!,
+ * Domains.jl:883-895) and floored at θ_r + eps, its effective saturation,
+ * then soil_conductance(S_l, S_c, d_ds, evap_p, evap_α, D_vapor) =
+ * D_vapor / max(dry_soil_layer_thickness(S_l, evap_α S_c, d_ds, evap_p), eps)
+ * (energy_hydrology.jl:1103-1131, 1322-1368), in m/s; dry_layer (may be NULL)
+ * receives the dry-layer thickness (m).  The turbulent fluxes that use it
+ * (SurfaceFluxes.jl) stay with the caller. */
+int clm_soil_conductance(clm_ctx* ctx, double d_ds, double evap_p, double evap_alpha,
+                         double D_vapor, double* g_soil, double* dry_layer);
 
 /* ---- group-solver shapes (CLM_SOLVER_GROUP) --------------------------------
  * Columns are bucketed by depth (max layers 8, 16, 24, 32, 40, 48, 64, 80,

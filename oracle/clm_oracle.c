@@ -280,6 +280,26 @@ double orc_phase_change_source(double theta_l, double theta_i, double T, double 
     return (theta_l - thstar) / tau;
 }
 
+/* retention_models.jl:30-44: S_c = (1 + ((n-1)/n)^(1-2n))^(-m), m = 1 - 1/n */
+double orc_vg_sc(double n) {
+    double m = 1.0 - 1.0 / n;
+    return POW(1.0 + POW((n - 1.0) / n, 1.0 - 2.0 * n), -m);
+}
+
+/* energy_hydrology.jl:1352-1368: αS_c clamped to [0, 1]; d_ds ((αS_c - S_l)/S_l)^p
+ * below it, 0 otherwise */
+double orc_dry_soil_layer_thickness(double S_l, double aSc, double d_ds, double p) {
+    aSc = fmin(fmax(aSc, 0.0), 1.0);
+    return S_l < aSc ? d_ds * POW((aSc - S_l) / S_l, p) : 0.0;
+}
+
+/* energy_hydrology.jl:1322-1350: D_vapor / max(dsl(S_l, α S_c, d_ds, p), eps) */
+double orc_soil_conductance(double S_l, double S_c, double d_ds, double p, double alpha,
+                            double D_vapor) {
+    double dsl = orc_dry_soil_layer_thickness(S_l, alpha * S_c, d_ds, p);
+    return D_vapor / fmax(dsl, DBL_EPSILON);
+}
+
 /* ---- column geometry (Domains.jl:935-949; ClimaCore center = face
  * midpoint, interior face J = center spacing, boundary face J = 2x the
  * center-to-boundary distance) -------------------------------------- */
@@ -717,6 +737,71 @@ static void eh_jacobian(const col_geom* g, const col_par* q, const double* vl,
     picard_tridiag(g, kap, tmpD, dtg, Wee);
 }
 
+/* The exact Jacobian of the same EnergyHydrology residual, for the Newton
+ * form of the inexact Newton (ORC_JAC_NEWTON; no counterpart in the
+ * reference).  R(x) = U0 + Δt T_imp(x) - x with T_imp of
+ * energy_hydrology.jl:363-425, whose x-dependence is the implicit cache
+ * (:427-445): ψ(ϑ_l; ν - θ_i) and T = T_0 + (ρe + θ_i ρ_i L_f)/ρc_s(θ_l, θ_i)
+ * with θ_l = min(ν - θ_i, ϑ_l).  K, κ and the boundary fluxes are lagged.
+ * With h = ψ + z, E = ρe_l(T) K, E' = ρ_l c_l K and per interior face f
+ * (cells f-1, f; gh = (h_f - h_{f-1})/Δz_f):
+ *   G_f = -κ_f (T_f - T_{f-1})/Δz_f - ½(E_{f-1} + E_f) gh
+ *   ∂G_f/∂T_{f-1} = κ_f/Δz_f - ½E'_{f-1} gh,  ∂G_f/∂T_f = -κ_f/Δz_f - ½E'_f gh
+ *   ∂G_f/∂h_{f-1} = ½(E_{f-1}+E_f)/Δz_f = -∂G_f/∂h_f
+ *   ∂T/∂ρe = 1/ρc_s,  ∂T/∂ϑ_l = -(T - T_0) ρ_l c_l/ρc_s while ϑ_l < ν - θ_i, else 0
+ * The water block equals the reference's (K is lagged, so dψ/dϑ is its
+ * whole derivative) and ∂R_ϑ/∂ρe = 0, so the system stays block lower
+ * triangular; W_ew carries no "- I" (the reference's A_ρϑ does, :554-556). */
+static void eh_jacobian_newton(const col_geom* g, const col_par* q, const double* vl,
+                               const double* ti, const double* K, const double* T,
+                               const double* kap, const double* psi, double dtg, double* Www,
+                               double* Wee, double* Wew, double* tmp /* 5N */) {
+    const int N = g->N;
+    double *dpsi = tmp, *dTdv = tmp + N, *dTde = tmp + 2 * N, *E = tmp + 3 * N,
+           *h = tmp + 4 * N;
+    for (int j = 0; j < N; ++j) {
+        const double nue = q->nu - ti[j];
+        const double rc = orc_volumetric_heat_capacity(fmin(nue, vl[j]), ti[j], q->rhoc_ds);
+        dTde[j] = 1.0 / rc;
+        dTdv[j] = vl[j] < nue ? -(T[j] - T_0) * RHO_L * CP_L / rc : 0.0;
+        dpsi[j] = orc_dpsidtheta(q->alpha, q->n, vl[j], nue, q->theta_r, q->Ss);
+        E[j] = orc_volumetric_internal_energy_liq(T[j]) * K[j];
+        h[j] = psi[j] + zc(g, j);
+    }
+    for (int i = 0; i < 3 * N; ++i) Www[i] = Wee[i] = Wew[i] = 0.0;
+    for (int f = 1; f < N; ++f) {
+        const double d = dzf(g, f);
+        const double Kf = 0.5 * (K[f - 1] + K[f]), kf = 0.5 * (kap[f - 1] + kap[f]);
+        const double gh = (h[f] - h[f - 1]) / d, Eb = 0.5 * (E[f - 1] + E[f]);
+        const double dF_lo = Kf / d * dpsi[f - 1], dF_hi = -Kf / d * dpsi[f];
+        const double dG_Tlo = kf / d - 0.5 * RHO_L * CP_L * K[f - 1] * gh;
+        const double dG_Thi = -kf / d - 0.5 * RHO_L * CP_L * K[f] * gh;
+        const double dG_vlo = dG_Tlo * dTdv[f - 1] + Eb / d * dpsi[f - 1];
+        const double dG_vhi = dG_Thi * dTdv[f] - Eb / d * dpsi[f];
+        const double dG_elo = dG_Tlo * dTde[f - 1], dG_ehi = dG_Thi * dTde[f];
+        /* face f is the upper face of cell f-1 (T_imp gets -G_f/Δz) and the
+         * lower face of cell f (+G_f/Δz) */
+        double s = -dtg / dzc(g, f - 1);
+        double* w = Www + 3 * (f - 1);
+        w[1] += s * dF_lo, w[2] += s * dF_hi;
+        w = Wew + 3 * (f - 1);
+        w[1] += s * dG_vlo, w[2] += s * dG_vhi;
+        w = Wee + 3 * (f - 1);
+        w[1] += s * dG_elo, w[2] += s * dG_ehi;
+        s = dtg / dzc(g, f);
+        w = Www + 3 * f;
+        w[0] += s * dF_lo, w[1] += s * dF_hi;
+        w = Wew + 3 * f;
+        w[0] += s * dG_vlo, w[1] += s * dG_vhi;
+        w = Wee + 3 * f;
+        w[0] += s * dG_elo, w[1] += s * dG_ehi;
+    }
+    for (int i = 0; i < N; ++i) {
+        Www[3 * i + 1] -= 1.0;
+        Wee[3 * i + 1] -= 1.0;
+    }
+}
+
 /* BlockLowerTriangularSolve(@name(soil.ϑ_l)) (implicit_timestepping.jl:161-169):
  * x_ϑ = A_ϑϑ \ b_ϑ;  x_ρe = A_ρρ \ (b_ρe - A_ρϑ x_ϑ);  -I blocks: x = -b. */
 static void eh_solve_col(int N, const double* Www, const double* Wee, const double* Wew,
@@ -869,7 +954,10 @@ static int eh_step_col(const orc_model* m, int64_t c, col_state* Y, double f_top
     }
     int k;
     for (k = 1; k <= max_iters; ++k) {
-        eh_jacobian(&g, &q, Y->vl, Y->ti, K, T, kap, thl, dt, Www, Wee, Wew, tmpD, tmpK);
+        if (m->jacobian == ORC_JAC_NEWTON)
+            eh_jacobian_newton(&g, &q, Y->vl, Y->ti, K, T, kap, psi, dt, Www, Wee, Wew, tmpD);
+        else
+            eh_jacobian(&g, &q, Y->vl, Y->ti, K, T, kap, thl, dt, Www, Wee, Wew, tmpD, tmpK);
         double diw, die;
         eh_imp_tendency(&g, K, psi, T, kap, tw, th, bw, bh, Rv, Ri, Re, &diw, &die);
         if (topm) eh_runoff_sink(N, sv, se, rss, ress, Rv, Re, &diw, &die);
@@ -1075,8 +1163,12 @@ int orc_jacobian(const orc_model* m, const orc_state* Y, const orc_aux* p, doubl
             col_geom g = {m->col_ptr[c + 1] - o, m->z_face + o + c};
             col_par q;
             load_par(m, c, &q);
-            double* t = scratch_get(&S, 2 * g.N + 2);
-            if (m->model == ORC_MODEL_RICHARDS) {
+            double* t = scratch_get(&S, 5 * g.N + 2);
+            if (m->model == ORC_MODEL_ENERGY_HYDROLOGY && m->jacobian == ORC_JAC_NEWTON) {
+                eh_jacobian_newton(&g, &q, y.vl, y.ti, a.K, a.T, a.kap, a.psi, dtgamma,
+                                   W_ww + 3 * (int64_t)o, W_ee + 3 * (int64_t)o,
+                                   W_ew + 3 * (int64_t)o, t);
+            } else if (m->model == ORC_MODEL_RICHARDS) {
                 for (int i = 0; i < g.N; ++i)
                     t[i] = orc_dpsidtheta(q.alpha, q.n, y.vl[i], q.nu, q.theta_r, q.Ss);
                 picard_tridiag(&g, a.K, t, dtgamma, W_ww + 3 * (int64_t)o);
@@ -1195,6 +1287,34 @@ int64_t orc_run(const orc_model* m, orc_state* Y, const double* top_water_tab,
         free(S.buf);
     }
     return err ? -3 : nonconv;
+}
+
+int orc_surface_soil_conductance(const orc_model* m, const orc_state* Y, double d_ds,
+                                 double evap_p, double evap_alpha, double D_vapor, double* g,
+                                 double* dsl) {
+    if (m->model != ORC_MODEL_ENERGY_HYDROLOGY) return -1;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < m->ncol; ++c) {
+        int32_t o = m->col_ptr[c];
+        col_geom gm = {m->col_ptr[c + 1] - o, m->z_face + o + c};
+        col_par q;
+        load_par(m, c, &q);
+        const int N = gm.N;
+        /* p.soil.θ_l of the top two cells (volumetric_liquid_fraction, ν - θ_i) */
+        double f1 = orc_volumetric_liquid_fraction(Y->vartheta_l[o + N - 1],
+                                                   q.nu - Y->theta_i[o + N - 1], q.theta_r);
+        double f2 = orc_volumetric_liquid_fraction(Y->vartheta_l[o + N - 2],
+                                                   q.nu - Y->theta_i[o + N - 2], q.theta_r);
+        double z1 = zc(&gm, N - 1), z2 = zc(&gm, N - 2);
+        /* linear_interpolation_to_surface! (Domains.jl:883-895) */
+        double th = (f1 - f2) / (z1 - z2) * (dztop(&gm) + z1 - z2) + f2;
+        th = fmax(th, q.theta_r + DBL_EPSILON);
+        double S = orc_effective_saturation(q.nu, th, q.theta_r);
+        double Sc = orc_vg_sc(q.n);
+        g[c] = orc_soil_conductance(S, Sc, d_ds, evap_p, evap_alpha, D_vapor);
+        if (dsl) dsl[c] = orc_dry_soil_layer_thickness(S, evap_alpha * Sc, d_ds, evap_p);
+    }
+    return 0;
 }
 
 int orc_totals(const orc_model* m, const orc_state* Y, double* water, double* energy) {

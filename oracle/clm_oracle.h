@@ -74,8 +74,14 @@ typedef struct {
     const double* z_face;
     const double* params;
     int32_t bc_top; /* ORC_BC_FLUX or ORC_BC_MOISTURE_STATE (RRE) */
-    int32_t reserved_;
+    /* Newton matrix of EnergyHydrology: ORC_JAC_PICARD, the reference's
+     * modified-Picard Jacobian (energy_hydrology.jl:466-576), or
+     * ORC_JAC_NEWTON, the exact derivative of the same residual (no
+     * counterpart in the reference; pinned by finite differences) */
+    int32_t jacobian;
 } orc_model;
+
+enum { ORC_JAC_PICARD = 0, ORC_JAC_NEWTON = 1 };
 
 /* prognostic state Y (EH uses all five; RRE uses vartheta_l, intF_water) */
 typedef struct {
@@ -158,6 +164,13 @@ double orc_phase_change_source(double theta_l, double theta_i, double T, double 
                                int* err);
 double orc_heaviside(double x);
 
+/* vanGenuchten S_c (retention_models.jl:30-44) and the surface vapour
+ * conductance closures (energy_hydrology.jl:1322-1368) */
+double orc_vg_sc(double n);
+double orc_dry_soil_layer_thickness(double S_l, double aSc, double d_ds, double p);
+double orc_soil_conductance(double S_l, double S_c, double d_ds, double p, double alpha,
+                            double D_vapor);
+
 /* ---- batch hooks (OpenMP over columns) ---- */
 /* make_update_cache: update_aux + update_boundary_fluxes at (Y, t) */
 int orc_update_cache(const orc_model* m, const orc_state* Y, const orc_forcing* f,
@@ -207,6 +220,11 @@ int64_t orc_run(const orc_model* m, orc_state* Y, const double* top_water_tab,
 /* totals: Σ ϑ Δz (+ρ_i/ρ_l θ_i Δz), Σ ρe Δz  (rre.jl:502-511,
  * energy_hydrology.jl:1282-1320) */
 int orc_totals(const orc_model* m, const orc_state* Y, double* water, double* energy);
+/* surface soil conductance per column of the state Y (EnergyHydrology):
+ * energy_hydrology.jl:1103-1131 with Domains.jl:883-895; dsl may be NULL */
+int orc_surface_soil_conductance(const orc_model* m, const orc_state* Y, double d_ds,
+                                 double evap_p, double evap_alpha, double D_vapor, double* g,
+                                 double* dsl);
 
 #ifdef __cplusplus
 }

@@ -28,7 +28,10 @@ CONSTANT_NAMES = ("RHO_L", "RHO_I", "CP_L", "CP_I", "T_0", "T_FREEZE", "LH_V0", 
 class Model(C.Structure):
     _fields_ = [("model", C.c_int32), ("bc_bottom", C.c_int32), ("ncol", C.c_int64),
                 ("col_ptr", _ip), ("z_face", _dp), ("params", _dp), ("bc_top", C.c_int32),
-                ("reserved_", C.c_int32)]
+                ("jacobian", C.c_int32)]
+
+
+JAC_PICARD, JAC_NEWTON = 0, 1  # EnergyHydrology Newton matrix (clm_oracle.h)
 
 
 BC_FLUX, BC_FREE_DRAINAGE, BC_MOISTURE_STATE, BC_TOPMODEL, BC_SURFACE_RUNOFF = 0, 1, 2, 3, 4
@@ -95,6 +98,9 @@ def lib(variant: str = ""):
             "orc_soil_Tf_depressed": [d, d, d, d, d, d],
             "orc_thermal_time": [d, d, d],
             "orc_heaviside": [d],
+            "orc_vg_sc": [d],
+            "orc_dry_soil_layer_thickness": [d, d, d, d],
+            "orc_soil_conductance": [d, d, d, d, d, d],
         }.items():
             f = getattr(L, name)
             f.argtypes = args
@@ -119,6 +125,7 @@ def lib(variant: str = ""):
                               C.c_int64, C.c_int]
         L.orc_run.restype = C.c_int64
         L.orc_totals.argtypes = [M, S, _dp, _dp]
+        L.orc_surface_soil_conductance.argtypes = [M, S, d, d, d, d, _dp, _dp]
         _libs[path] = L
     return _libs[path]
 
@@ -140,7 +147,7 @@ class OracleModel:
     """Oracle view of a CSR column batch (same arrays the library takes)."""
 
     def __init__(self, model: int, bc_bottom: int, col_ptr, z_face, params, variant="",
-                 bc_top: int = BC_FLUX):
+                 bc_top: int = BC_FLUX, jacobian: int = JAC_PICARD):
         self.L = lib(variant)
         self.col_ptr = np.ascontiguousarray(col_ptr, dtype=np.int32)
         self.z_face = np.ascontiguousarray(z_face, dtype=np.float64)
@@ -150,12 +157,12 @@ class OracleModel:
         self.ncol = self.col_ptr.size - 1
         self.ncell = int(self.col_ptr[-1])
         self._m = Model(model, bc_bottom, self.ncol, self.col_ptr.ctypes.data_as(_ip),
-                        _p(self.z_face), _p(self.params), bc_top, 0)
+                        _p(self.z_face), _p(self.params), bc_top, jacobian)
 
     @classmethod
-    def from_batch(cls, b, variant=""):
+    def from_batch(cls, b, variant="", jacobian: int = JAC_PICARD):
         return cls(b.model, b.bc_bottom, b.col_ptr, b.z_face, b.params, variant=variant,
-                   bc_top=getattr(b, "bc_top", BC_FLUX))
+                   bc_top=getattr(b, "bc_top", BC_FLUX), jacobian=jacobian)
 
     def new_state(self, src: dict | None = None) -> dict:
         st = {"vartheta_l": np.zeros(self.ncell), "intF_water": np.zeros(self.ncol)}
@@ -264,6 +271,16 @@ class OracleModel:
         if nc < 0:
             raise ValueError("Matric potential is positive")
         return it, int(nc)
+
+    def surface_soil_conductance(self, Y: dict, d_ds: float, evap_p: float, evap_alpha: float,
+                                 D_vapor: float):
+        g = np.zeros(self.ncol)
+        dsl = np.zeros(self.ncol)
+        r = self.L.orc_surface_soil_conductance(C.byref(self._m), C.byref(self._s(Y)), d_ds,
+                                                evap_p, evap_alpha, D_vapor, _p(g), _p(dsl))
+        if r:
+            raise ValueError("surface soil conductance is defined for EnergyHydrology")
+        return g, dsl
 
     def totals(self, Y: dict):
         w = np.zeros(self.ncol)

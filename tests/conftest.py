@@ -48,3 +48,31 @@ def column_ok(batch, Y, vmax=5.0):
     if "rho_e_int" in Y and Y.get("rho_e_int") is not None:
         ok &= np.isfinite(Y["rho_e_int"]) & np.isfinite(Y["theta_i"])
     return np.minimum.reduceat(ok.astype(np.int8), batch.col_ptr[:-1]).astype(bool)
+
+
+STALL_RESIDUAL = 1e-3  # the library's stall bound (ϑ_l; ρe_int/ρc_s in kelvin)
+
+
+def oracle_exit_residual(o, batch, Yn, Yx, slot):
+    """Per column, max |R(x)| of the step from Yn that ended at Yx, with the
+    cache of u_n for the lagged fields and boundary fluxes and the implicit
+    cache at x (R = U0 + Δt T_imp(x) - x; EnergyHydrology's ρe_int row scaled
+    by ρc_s).  The library flags a column-step stalled when this residual of
+    its last iteration exceeds STALL_RESIDUAL after the increments met tol."""
+    w = batch.workload
+    eh = batch.model == 1
+    th = batch.top_heat[slot].copy() if eh else None
+    a = o.new_aux()
+    o.update_cache(Yn, a, batch.top_water[slot].copy(), th)
+    U0 = {k: v.copy() for k, v in Yn.items()}
+    if eh:
+        e = o.exp_tendency(Yn, a)
+        for k in U0:
+            U0[k] = Yn[k] + w.dt * e[k]
+    o.update_implicit_cache(Yx, a)
+    d = o.imp_tendency(Yx, a)
+    R = np.abs(U0["vartheta_l"] + w.dt * d["vartheta_l"] - Yx["vartheta_l"])
+    if eh:
+        rc = np.repeat(batch.params[12], batch.nlayers) + 4.181e6 * np.clip(Yx["vartheta_l"], 0, 1)
+        R = np.maximum(R, np.abs(U0["rho_e_int"] + w.dt * d["rho_e_int"] - Yx["rho_e_int"]) / rc)
+    return np.maximum.reduceat(np.where(np.isfinite(R), R, np.inf), batch.col_ptr[:-1])

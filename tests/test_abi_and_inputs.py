@@ -36,9 +36,9 @@ def test_library_exports_every_declared_symbol():
 def test_header_enums_match_binding():
     from clmsoil import _lib
     txt = open(HEADER).read()
-    assert "CLM_ABI_VERSION 2" in txt
+    assert "CLM_ABI_VERSION 3" in txt
     lib = _lib.load()
-    assert lib.clm_abi_version() == 2
+    assert lib.clm_abi_version() == 3
     assert "CLM_NPARAM_RRE 6" in txt and _lib.NPARAM_RRE == 6 and _lib.NPARAM_EH == 13
     block = txt.split("CLM_P_NU = 0,")[1].split("};")[0]
     order = re.findall(r"CLM_P_(\w+)", block)
@@ -47,13 +47,34 @@ def test_header_enums_match_binding():
                                    "KAPPA_SAT_FROZEN", "KAPPA_SAT_UNFROZEN", "RHOC_DS"]
 
 
-def test_struct_layouts_match_header():
+def test_struct_layouts_match_header(tmp_path):
+    """Size and every field offset of the ctypes mirrors against what a C
+    compiler makes of include/clm_soil.h."""
+    import subprocess
     from clmsoil import _lib
-    assert C.sizeof(_lib.Config) == 4 + 4 + 8 + 8 * 3 + 4 * 3 + 4 + 8 + 4 + 4  # incl. padding
-    assert C.sizeof(_lib.State) == 5 * 8
-    assert C.sizeof(_lib.Aux) == 13 * 8
-    assert C.sizeof(_lib.SurfaceFluxes) == 7 * 8
-    assert C.sizeof(_lib.StepStats) == 4 + 4 + 8 + 8 * 3 + 8
+    structs = {"clm_config": _lib.Config, "clm_state": _lib.State, "clm_aux": _lib.Aux,
+               "clm_surface_fluxes": _lib.SurfaceFluxes, "clm_step_stats": _lib.StepStats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "clm_soil.h"',
+             'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True,
+                             text=True).stdout.splitlines():
+        cname, field, val = ln.split()
+        got[(cname, field)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
 
 
 @pytest.mark.parametrize("mutate,code", [
@@ -63,6 +84,8 @@ def test_struct_layouts_match_header():
     (lambda c: setattr(c, "jac_precision", 16), -1),
     (lambda c: setattr(c, "newton_max_iters", 0), -1),
     (lambda c: setattr(c, "solver", 9), -1),
+    (lambda c: setattr(c, "jacobian", 5), -1),
+    (lambda c: setattr(c, "jacobian", 1), -1),        # the Newton matrix is EH only
     (lambda c: setattr(c, "bc_top", 1), -1),          # free drainage is a bottom condition
     (lambda c: setattr(c, "bc_top", 7), -1),
     (lambda c: (setattr(c, "model", 1), setattr(c, "nparam", 13), setattr(c, "bc_top", 2)), -1),

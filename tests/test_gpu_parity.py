@@ -715,3 +715,95 @@ def test_forcing_and_state_shapes_are_checked(gpu_lib):
     for k, v in sim.state().items():
         assert np.array_equal(v, Ya[k], equal_nan=True), k
     m.close()
+
+
+# -------------------------------------- the learned forms against the oracle
+
+def _train_warm_start(name, jac, jacobian=0, ncol=8192, nsteps=12, epochs=8):
+    """Train the warm start the way bench.py does, smaller: plain steps on
+    columns of other seeds give the (state, converged step) pairs."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
+    bt = inputs.make_batch(inputs.training_workload(name), ncol=ncol, nsteps=nsteps)
+    w = bt.workload
+    m = model_from_batch(bt, jac_precision=jac, solver=1, jacobian=jacobian)
+    ws = NewtonWarmStart(m, seed=5)
+    sim = LandSimulation(m, bt.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                         tol=w.newton_tol)
+    ws.collect(sim, bt.top_water, bt.top_heat, nsteps, w.steps_per_slot)
+    losses = ws.train(epochs=epochs, batch=2048, lr=1e-3, seed=9)
+    params = ws.params()
+    m.close()
+    return params, losses
+
+
+@pytest.mark.parametrize("name,jac,jacobian", [
+    pytest.param("eh_262144_uneven", 32, 1, id="eh_learned_newton"),
+    pytest.param("eh_262144_uneven", 32, 0, id="eh_learned_mixed"),
+    pytest.param("rre_131072_uneven", 64, 0, id="rre_learned")])
+def test_learned_form_per_step_vs_fp64_oracle(gpu_lib, name, jac, jacobian):
+    """The learned forms, trained as bench.py trains them, as a per-step map
+    against the FP64 oracle (the reference's method) on 512 columns x 24
+    steps: every column-step that converges and stays finite on both sides
+    (and that the oracle does not leave stalled, see oracle_exit_residual)
+    is within 1e-6 in ϑ_l and 1e-6 K in ρe_int/ρc_s."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
+    from oracle.oracle import OracleModel
+    from conftest import STALL_RESIDUAL, oracle_exit_residual
+    params, losses = _train_warm_start(name, jac, jacobian)
+    print(f"{name}: warm start trained, loss {losses[0]:.3g} -> {losses[-1]:.3g}", flush=True)
+    b = inputs.make_batch(name, ncol=512, nsteps=24)
+    w = b.workload
+    eh = b.model == 1
+    m = model_from_batch(b, jac_precision=jac, solver=1, jacobian=jacobian)
+    ws = NewtonWarmStart(m, seed=0)
+    ws.set_params(params)
+    ws.enable(True)
+    sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters, tol=w.newton_tol)
+    o = OracleModel.from_batch(b)
+    Y = b.copy_state()
+    held = total = nstall_o = nstall_g = nfb = 0
+    wv = we = 0.0
+    kg, ko = [], []
+    for s in range(24):
+        sl = b.forcing_slot(s)
+        th = b.top_heat[sl:sl + 1] if eh else None
+        sim.reset(Y)
+        _, it = sim.solve(b.top_water[sl:sl + 1], th, 1, 1, want_iters=True)
+        it = it[0]
+        Yg = sim.state()
+        Yo = {k: v.copy() for k, v in Y.items()}
+        ito, _ = o.step(Yo, w.dt, w.newton_max_iters, w.newton_tol, b.top_water[sl].copy(),
+                        None if th is None else th[0].copy())
+        r_o = oracle_exit_residual(o, b, Y, Yo, sl)
+        stall_o = (ito < w.newton_max_iters) & (r_o > STALL_RESIDUAL)
+        nstall_o += stall_o.sum()
+        nstall_g += ((it & 0x100000) != 0).sum()
+        nfb += ((it & 0x80000) != 0).sum()
+        ok = column_ok(b, Y) & column_ok(b, Yg) & column_ok(b, Yo)
+        conv = ok & ((it & 0x30000) == 0) & (ito < w.newton_max_iters) & ~stall_o
+        cm = _cell_mask(b, conv)
+        dv = np.abs(Yg["vartheta_l"] - Yo["vartheta_l"])
+        bad = dv > 1e-6
+        wv = max(wv, dv[cm].max(initial=0.0))
+        if eh:
+            rc = np.repeat(b.params[12], b.nlayers) + 4.181e6 * np.clip(Yo["vartheta_l"], 0, 1)
+            de = np.abs(Yg["rho_e_int"] - Yo["rho_e_int"]) / rc
+            bad |= de > 1e-6
+            we = max(we, de[cm].max(initial=0.0))
+        colbad = np.maximum.reduceat(np.where(cm & bad, 1, 0), b.col_ptr[:-1]) > 0
+        held += (conv & ~colbad).sum()
+        total += conv.sum()
+        kg.append((it & 0xFFFF)[conv])
+        ko.append(ito[conv])
+        Y = Yo
+    kg, ko = np.concatenate(kg), np.concatenate(ko)
+    print(f"{name} learned (jac {jac}, jacobian {jacobian}): {held} of {total} converged "
+          f"column-steps ({total / (24 * 512):.3f} of all) within 1e-6 of the FP64 oracle; max "
+          f"|dϑ| {wv:.2e}, |dρe|/ρc {we:.2e} K; mean Newton iters {kg.mean():.2f} vs oracle "
+          f"{ko.mean():.2f}; stalled: oracle {nstall_o}, GPU {nstall_g}; fallbacks {nfb}",
+          flush=True)
+    assert total >= 0.7 * 24 * 512
+    assert held == total
+    m.close()
