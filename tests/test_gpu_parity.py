@@ -807,3 +807,45 @@ def test_learned_form_per_step_vs_fp64_oracle(gpu_lib, name, jac, jacobian):
     assert total >= 0.7 * 24 * 512
     assert held == total
     m.close()
+
+
+def test_stall_flag_matches_oracle_residual(gpu_lib):
+    """Plain FP64 RichardsModel, 4,096 columns x 6 steps as a per-step map:
+    the GPU flags a column-step stalled (0x100000, also counted as not
+    converged) exactly where the FP64 oracle's increments met tol with a
+    last residual above 1e-3 -- decided on residuals of order 1e6 or more
+    against 1e-4 or less, so rounding does not move a column across."""
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation
+    from oracle.oracle import OracleModel
+    from conftest import STALL_RESIDUAL, oracle_exit_residual
+    b = inputs.make_batch("rre_131072_uneven", ncol=4096, nsteps=6)
+    w = b.workload
+    m, o = _models(b, jac=64, solver=1)
+    sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters, tol=w.newton_tol)
+    Y = b.copy_state()
+    n_o = n_g = n_both = 0
+    for s in range(6):
+        sl = b.forcing_slot(s)
+        sim.reset(Y)
+        st, it = sim.solve(b.top_water[sl:sl + 1], None, 1, 1, want_iters=True)
+        it = it[0]
+        Yo = {k: v.copy() for k, v in Y.items()}
+        ito, _ = o.step(Yo, w.dt, w.newton_max_iters, w.newton_tol, b.top_water[sl].copy())
+        r = oracle_exit_residual(o, b, Y, Yo, sl)
+        fin = column_ok(b, Yo, vmax=1e300)
+        so = fin & (ito < w.newton_max_iters) & (r > STALL_RESIDUAL)
+        sg = (it & 0x100000) != 0
+        assert ((it[sg] & 0x10000) != 0).all()
+        assert st.n_stalled == sg.sum()
+        # the oracle's FP64 iterate and the GPU's take the same path where
+        # both are finite and converge in the same count
+        same = fin & ((it & 0xFFFF) == ito)
+        assert (so[same] == sg[same]).all(), (s, np.nonzero(so[same] != sg[same]))
+        n_o += so.sum()
+        n_g += sg.sum()
+        n_both += (so & sg).sum()
+        Y = Yo
+    print(f"stalls: oracle {n_o}, GPU {n_g}, both {n_both} of {6 * b.ncol} column-steps",
+          flush=True)
+    assert n_both > 0
