@@ -104,10 +104,16 @@ __host__ __device__ __forceinline__ double clm_log(double x) {
     return (x > 0.0 && x < INFINITY) ? y : sp;
 }
 
+// y log x with 0 log x = 0 (so x^0 = 1 also for x = 0, as in the reference's
+// ^): the exponent of a factor that is absent from a product of powers
+__host__ __device__ __forceinline__ double mul_log(double y, double log_x) {
+    return y == 0.0 ? 0.0 : y * log_x;
+}
+
 // x^y = exp(y log x) for x >= 0: relative error about |y log x| ulp, which
 // the closures' bases and exponents keep below ~50 ulp (1e-14).
 __host__ __device__ __forceinline__ double clm_pow(double x, double y) {
-    return exp(y * clm_log(x));
+    return exp(mul_log(y, clm_log(x)));
 }
 
 // per-column van Genuchten constants, derived once per column
@@ -302,9 +308,11 @@ __device__ __forceinline__ double thermal_cond(double thl, double thi, double nu
     double p = unfrozen ? (1.0 + om - KERSTEN_A * qz - gr) / 2.0 : 1.0 + om;
     double q = unfrozen ? 1.0 - om : 0.0;
     double g = unfrozen ? t1 - h * h * h : 1.0;
-    double Ke = exp(p * clm_log(Sr) + q * clm_log(g));
+    double Ke = exp(mul_log(p, clm_log(Sr)) + mul_log(q, clm_log(g)));
     double tw = thl + thi;
-    double ks = (tw < DEPS) ? (k_satu + k_satf) / 2.0 : exp((thl * lk_u + thi * lk_f) / tw);
+    // κ_f^0 = 1 also for κ_f = 0 (an ice-free cell of a column with κ = 0)
+    double ks = (tw < DEPS) ? (k_satu + k_satf) / 2.0
+                            : exp((mul_log(thl, lk_u) + mul_log(thi, lk_f)) / tw);
     return Ke * ks + (1.0 - Ke) * k_dry;
 }
 
