@@ -649,6 +649,18 @@ class LandSimulation:
         self.t += self.Δt
         return st
 
+    def step_noalloc(self) -> L.StepStats:
+        """step!(sim) through clm_step_noalloc: the same step with its Newton
+        flags and statistics in the library's static device scratch, so the
+        call allocates nothing (state and forcing go up first if the host
+        copy changed, as in step)."""
+        m = self.model
+        self._push()
+        st = L.StepStats()
+        m._check(m._lib.clm_step_noalloc(m._ctx, self.t, self.Δt, C.byref(st)))
+        self.t += self.Δt
+        return st
+
     def save_checkpoint(self, output_dir: str) -> str:
         """ClimaLand.save_checkpoint(Y, t, output_dir): the device state and
         time to <output_dir>/day<D>.<S>.clmchk; returns the file's path."""

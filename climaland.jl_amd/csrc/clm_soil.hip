@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -30,7 +31,14 @@ struct DevStats {
     unsigned int pad;
 };
 
+constexpr int64_t kScratchFlags = int64_t(1) << 20;
+constexpr size_t kScratchHead = 256;  // DevStats, padded
+static_assert(sizeof(DevStats) <= kScratchHead, "statistics record fits the head");
+
 }  // namespace
+
+// clm_step_noalloc's scratch (its layout and use: scratch_address below)
+static __device__ __align__(256) unsigned char scratch[kScratchHead + 4 * kScratchFlags];
 
 struct clm_ctx {
     int model = 0, bc_bottom = 0, bc_top = 0, jac_prec = 64, max_iters = 3;
@@ -110,6 +118,8 @@ struct clm_ctx {
     double g_dt = 0.0;
     double last_ms = 0.0;
     int32_t last_launches = 0;
+    // device address of this GPU's static scratch (clm_step_noalloc)
+    unsigned char* d_scratch = nullptr;
     std::string err;
 };
 
@@ -996,6 +1006,36 @@ void destroy_graph(clm_ctx* c) {
     c->g_nsteps = -1;
 }
 
+// The static scratch of clm_step_noalloc: the step's statistics record, then
+// one Newton flag word per column (group solver) or column slot (lane
+// solver).  Each GPU holds its own copy, at its own address, until the
+// process ends; every context on a GPU shares it.  In host code `scratch`
+// names a host-side stand-in, never the device memory: kernels and copies
+// take the address hipGetSymbolAddress returns for the current device.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// the current device's scratch address, looked up once per device
+int scratch_address(clm_ctx* c, unsigned char** out) {
+    int dev = -1;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_err(c, CLM_E_HIP, "device %d is beyond the %d the scratch table holds", dev,
+                       kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(c, CLM_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+                           dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CLM_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1306,6 +1346,10 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         set_err(c, CLM_E_HIP, "hipEventCreate failed");
         return fail(CLM_E_HIP);
+    }
+    {
+        int r = scratch_address(c, &c->d_scratch);
+        if (r) return fail(r);
     }
     *out = c;
     return CLM_OK;
@@ -1784,6 +1828,48 @@ int clm_step(clm_ctx* c, double t, double dt, clm_step_stats* stats) {
         c->last_ms = ms;
         c->last_launches = 1;
         if ((r = collect_stats(c, 1, stats))) return r;
+        stats->ms = ms;
+        if (stats->n_domain_error)
+            return set_err(c, CLM_E_DOMAIN,
+                           "inverse_matric_potential: matric potential is positive");
+    }
+    return CLM_OK;
+}
+
+// clm_step with its Newton flags and statistics in the GPU's static scratch:
+// the same kernels, and no allocation on any call (clm_step allocates its
+// flag table on first use).  Calls that overlap on one GPU race on the
+// scratch.  The step kernels write every column's flag word; the statistics
+// record is cleared by a kernel before the reduction.
+int clm_step_noalloc(clm_ctx* c, double t, double dt, clm_step_stats* stats) {
+    (void)t;
+    if (!c) return CLM_E_ARG;
+    if (!(dt > 0.0)) return set_err(c, CLM_E_ARG, "dt must be > 0");
+    if (!c->d_scratch) return set_err(c, CLM_E_STATE, "clm_step_noalloc: no scratch address");
+    if (ncolvec(c) > kScratchFlags)
+        return set_err(c, CLM_E_ARG,
+                       "clm_step_noalloc: %lld column slots exceed the scratch's %lld; use "
+                       "clm_step",
+                       (long long)ncolvec(c), (long long)kScratchFlags);
+    DevStats* dst = (DevStats*)c->d_scratch;
+    int32_t* iters = (int32_t*)(c->d_scratch + kScratchHead);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (is_group(c))
+        launch_step(c, c->d_cftw, c->d_cfth, dt, iters);
+    else
+        launch_step(c, c->d_ftw, c->d_fth, dt, iters);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->have_cache = false;
+    c->have_jac = false;
+    if (stats) {
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last_ms = ms;
+        c->last_launches = 1;
+        int r = collect_stats(c, 1, stats, iters, dst);
+        if (r) return r;
         stats->ms = ms;
         if (stats->n_domain_error)
             return set_err(c, CLM_E_DOMAIN,

@@ -278,6 +278,16 @@ int clm_step(clm_ctx* ctx, double t, double dt, clm_step_stats* stats);
 int clm_run(clm_ctx* ctx, double t0, double dt, int32_t nsteps, int32_t steps_per_slot,
             clm_step_stats* stats, int32_t* iters_out);
 
+/* clm_step without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same kernels and
+ * results as clm_step; the Newton flags and statistics live in a static
+ * device scratch of the library (one per GPU, shared by every context on
+ * it, kept until the process ends): at most 1,048,576 columns (group
+ * solver) or column slots (lane solver), else CLM_E_ARG.  Calls that overlap
+ * on one GPU race on that scratch. */
+int clm_step_noalloc(clm_ctx* ctx, double t, double dt, clm_step_stats* stats);
+
 /* Column totals Σ ϑ_l Δz (+ ρ_i/ρ_l θ_i Δz) and Σ ρe_int Δz of the current
  * device state (total_liq_water_vol_per_area!, rre.jl:502-511,
  * energy_hydrology.jl:1282-1320).  energy may be NULL. */
