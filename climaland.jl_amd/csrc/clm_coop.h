@@ -22,6 +22,7 @@
 #include <type_traits>
 
 #include "clm_eh.h"
+#include "clm_nodes.h"
 
 namespace clm {
 
@@ -369,7 +370,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
     double* __restrict__ xg, double* __restrict__ iw, const double* __restrict__ f_top,
     const double* __restrict__ f_bot, int bc, double dt, int max_iters, double tol,
-    int32_t* __restrict__ iters, const double* __restrict__ gv, double* __restrict__ u0out) {
+    int32_t* __restrict__ iters, const float* __restrict__ wso, double* __restrict__ u0out) {
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
     const int gl = id.gl;
@@ -379,7 +380,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     const int64_t base = col_ptr[col];
     const int N = (int)(col_ptr[col + 1] - base);
     const int64_t fbase = base + col;
-    const bool warm = gv != nullptr;
+    const bool warm = wso != nullptr;
     const VG v = make_vg(par[P_VG_ALPHA * ncol + col], par[P_VG_N * ncol + col],
                          par[P_THETA_R * ncol + col], par[P_SS * ncol + col],
                          par[P_KSAT * ncol + col]);
@@ -458,9 +459,11 @@ __global__ __launch_bounds__(256) void k_rre_coop(
     double err = 0.0;
     for (int attempt = 0; attempt < 2; ++attempt) {
     if (attempt == 0 && warm) {  // x0 = U0 + guess; cache_imp!(x0) happens at the loop top
+        // the guess at each cell from the column's network output row
+        const float* o = wso + (int64_t)col * ws::kNodes;
 #pragma unroll
         for (int k = 0; k < R; ++k)
-            if (on[k]) x[k] = u0[k] + gv[base + gl * R + k];
+            if (on[k]) x[k] = u0[k] + ws::at_cell(gl * R + k, N, o) * ws::kScaleV;
     }
     if (attempt == 1) {  // the warm start went non-finite or stalled: plain Newton from U0
         if (!(warm && (!isfinite(err) || stalled))) break;
@@ -596,7 +599,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     double* __restrict__ iw, double* __restrict__ ie, const double* __restrict__ ftw,
     const double* __restrict__ fth, const double* __restrict__ fbw,
     const double* __restrict__ fbh, int bc_bottom, double dt, int max_iters, double tol,
-    int32_t* __restrict__ iters, const double* __restrict__ gv, const double* __restrict__ ge,
+    int32_t* __restrict__ iters, const float* __restrict__ wso,
     double* __restrict__ u0out, int bc_top, Surface sf) {
     GroupId id = group_id<G>(ncols_bin);
     if (!id.live) return;
@@ -778,7 +781,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         else
             efac.factor(xc, ea, eb, ec);
     }
-    const bool warm = gv != nullptr;
+    const bool warm = wso != nullptr;
     int kit = 0, kit_total = 0;
     bool conv = false, fell_back = false, stalled = false;
     double err = 0.0;
@@ -786,12 +789,18 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     // the Newton matrices on the first attempt, the reference's on a fallback
     const bool nw = NEWTON && attempt == 0;
     if (attempt == 0 && warm) {  // x0 = U0 + guess, cache_imp!(x0) and Wfact(x0)
+        // the guess at each cell from the column's network output row: ϑ_l,
+        // then ρe_int / ρc_s(u_n) in kelvin, ρc_s at θ_l = min(ν - θ_i, ϑ_l) of
+        // u_n (re-read: HBM holds u_n until the end)
+        const float* o = wso + (int64_t)col * (2 * ws::kNodes);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             int r = gl * R + k;
             if (on[k]) {
-                xv[k] = u0v[k] + gv[base + r];
-                xe[k] = u0e[k] + ge[base + r];
+                const double v0 = xvg[base + r], t0 = xig[base + r];
+                xv[k] = u0v[k] + ws::at_cell(r, N, o) * ws::kScaleV;
+                xe[k] = u0e[k] + ws::at_cell(r, N, o + ws::kNodes) * ws::kScaleT *
+                                     heat_capacity(fmin(e.nu - t0, v0), t0, e.rhoc_ds);
             }
             if constexpr (NEWTON)
                 newton_T(e, xv[k], xi[k], xe[k], Tp[k], rcx[k], dTdv[k]);

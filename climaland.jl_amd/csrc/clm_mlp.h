@@ -21,6 +21,7 @@
 // the warm-started iterate turns non-finite.
 #pragma once
 #include "clm_eh.h"
+#include "clm_nodes.h"
 
 namespace clm {
 namespace ws {
@@ -30,36 +31,11 @@ typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
 
-constexpr int kNodes = 32;
-constexpr double kScaleV = 1e-2;  // ϑ_l output unit
-constexpr double kScaleT = 1.0;   // ρe/ρc_s output unit (K)
-
 struct Dims {
     int F, Fp, H, Nout;
 };
 __host__ __device__ inline Dims dims_for(bool eh) {
     return eh ? Dims{105, 128, 384, 64} : Dims{40, 64, 256, 32};
-}
-
-// ---- resampling between layers and nodes ------------------------------------
-// value at node j of a column with N cells (v: cell accessor)
-template <typename F>
-__device__ __forceinline__ double at_node(int j, int N, F v) {
-    double q = ((double)j + 0.5) * (double)N / (double)kNodes - 0.5;
-    q = fmin(fmax(q, 0.0), (double)(N - 1));
-    int i0 = (int)q;
-    int i1 = i0 + 1 < N ? i0 + 1 : N - 1;
-    double t = q - (double)i0;
-    return (1.0 - t) * v(i0) + t * v(i1);
-}
-// value at cell i of node values o[0..kNodes)
-__device__ __forceinline__ double at_cell(int i, int N, const float* __restrict__ o) {
-    double q = ((double)i + 0.5) * (double)kNodes / (double)N - 0.5;
-    q = fmin(fmax(q, 0.0), (double)(kNodes - 1));
-    int j0 = (int)q;
-    int j1 = j0 + 1 < kNodes ? j0 + 1 : kNodes - 1;
-    double t = q - (double)j0;
-    return (1.0 - t) * (double)o[j0] + t * (double)o[j1];
 }
 
 // Features are O(1) by construction.  A column whose iterate blew up enters
@@ -518,18 +494,19 @@ __global__ __launch_bounds__(512) void k_gemm_wide(
     }
 }
 
-// ---- fused inference: features -> 4 layers -> per-cell guess in one kernel -----
+// ---- fused inference: features -> 4 layers in one kernel -----------------------
 // A 512-thread block takes 128 columns.  Their feature rows are built in LDS
 // (the same lanes and arithmetic as k_features), pass through the four layers
 // with the activations kept in LDS (one [128][H + 8] BF16 buffer, overwritten
 // by each hidden layer's output once every wave has finished reading it), and
-// the output rows become per-cell guesses as in k_guess.  HBM sees the
-// state, the parameters and the guesses only.  Weights stream from L2 in
+// the output layer writes the FP32 output rows [ncol][Nout]; the step kernels
+// turn a column's row into its per-cell guesses as k_guess does.  HBM sees the
+// state, the parameters and the output rows only.  Weights stream from L2 in
 // chunk-major order ([K/32][N][32], written by k_cast_weights / k_adam) through
 // a double-buffered [2][N][32 + 8] LDS buffer, one barrier per 32-wide K chunk.
 // Each layer keeps the accumulation order of k_gemm_wide / k_gemm_nt (K chunks
 // of 32 ascending, two 16-deep MFMA steps per chunk, the same transposed
-// operand order), so the guesses equal the unfused path's bit for bit.
+// operand order), so the outputs equal the unfused path's bit for bit.
 constexpr int kIBM = 128;
 
 template <bool EH>
@@ -571,8 +548,8 @@ struct Chunk {
 };
 
 // one layer over the block's 128 rows: Act[:, 0:K] . W^T + bias; hidden layers
-// write GELU(.) as BF16 into Act[:, 0:N], the output layer writes FP32 to Y
-// ([128][N] in LDS, aliasing Wb once the K loop is done).  On entry chunk 0
+// write GELU(.) as BF16 into Act[:, 0:N], the output layer writes FP32 rows
+// [rows][N] to Y in global memory (rows past `rows` are not stored).  On entry chunk 0
 // of W is in Wb buffer 0 (behind a barrier); on exit, if NN > 0, chunk 0 of
 // the next layer's weights Wnext ([NN][K'] chunk-major) is, its load having
 // overlapped this layer's epilogue.
@@ -580,7 +557,8 @@ template <int N, int K, bool HIDDEN, int NN>
 __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                             bf16* __restrict__ Wb,
                                             const bf16* __restrict__ Wpk,
-                                            const float* __restrict__ bias, float* Y,
+                                            const float* __restrict__ bias,
+                                            float* __restrict__ Y, int rows,
                                             const bf16* __restrict__ Wnext) {
     constexpr int WN = N / 32 < 4 ? N / 32 : 4;          // waves across columns
     constexpr int WM = 8 / WN < 4 ? 8 / WN : 4;          // waves across rows
@@ -650,8 +628,8 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                         bf16x4 hv = {(bf16)gelu(v.x), (bf16)gelu(v.y), (bf16)gelu(v.z),
                                      (bf16)gelu(v.w)};
                         *(bf16x4*)(Act + m * lds_row + n) = hv;
-                    } else {
-                        *(float4*)(Y + m * N + n) = v;
+                    } else if (m < rows) {
+                        *(float4*)(Y + (int64_t)m * N + n) = v;
                     }
                 }
         }
@@ -667,13 +645,11 @@ __global__ __launch_bounds__(512, 1) void k_ws_infer(
     const double* __restrict__ xe, const double* __restrict__ ftw, const double* __restrict__ fth,
     double dt, const bf16* __restrict__ W0, const bf16* __restrict__ W1, const bf16* __restrict__ W2,
     const bf16* __restrict__ W3, const float* __restrict__ b0, const float* __restrict__ b1,
-    const float* __restrict__ b2, const float* __restrict__ b3, double* __restrict__ gv,
-    double* __restrict__ ge) {
+    const float* __restrict__ b2, const float* __restrict__ b3, float* __restrict__ Yg) {
     using D = NetDims<EH>;
     constexpr int LR = D::H + 8;  // LDS row stride of the activations
     __shared__ __attribute__((aligned(16))) bf16 Act[kIBM * LR];
     __shared__ __attribute__((aligned(16))) bf16 Wb[2 * D::H * kLdsRow];
-    static_assert(kIBM * D::No * sizeof(float) <= sizeof(Wb), "output rows alias Wb");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * kIBM;
     // features: each wave takes two columns per pass, lane j -> node j.  Rows
@@ -692,16 +668,11 @@ __global__ __launch_bounds__(512, 1) void k_ws_infer(
         first.store(Wb, 0);
     }
     __syncthreads();
-    float* Y = (float*)Wb;
-    infer_layer<D::H, D::Fp, true, D::H>(Act, LR, Wb, W0, b0, nullptr, W1);
-    infer_layer<D::H, D::H, true, D::H>(Act, LR, Wb, W1, b1, nullptr, W2);
-    infer_layer<D::H, D::H, true, D::No>(Act, LR, Wb, W2, b2, nullptr, W3);
-    infer_layer<D::No, D::H, false, 0>(Act, LR, Wb, W3, b3, Y, nullptr);
-#pragma unroll 4
-    for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
-        const int64_t c = c0 + lc;
-        if (c < ncol) guess_lane(EH, col_ptr, ncol, par, xv, xi, Y + lc * D::No, c, lane & 31, gv, ge);
-    }
+    const int rows = (int)min<int64_t>(kIBM, ncol - c0);
+    infer_layer<D::H, D::Fp, true, D::H>(Act, LR, Wb, W0, b0, nullptr, 0, W1);
+    infer_layer<D::H, D::H, true, D::H>(Act, LR, Wb, W1, b1, nullptr, 0, W2);
+    infer_layer<D::H, D::H, true, D::No>(Act, LR, Wb, W2, b2, nullptr, 0, W3);
+    infer_layer<D::No, D::H, false, 0>(Act, LR, Wb, W3, b3, Yg + c0 * D::No, rows, nullptr);
 }
 
 // ---- small training kernels ---------------------------------------------------

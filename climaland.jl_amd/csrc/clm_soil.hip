@@ -521,8 +521,7 @@ void launch_step_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
 // optional per-step extras of the group kernels: a Newton warm start
 // (x0 = U0 + guess) and an output copy of the predictor U0_ϑ
 struct StepExtra {
-    const double* gv = nullptr;
-    const double* ge = nullptr;
+    const float* wso = nullptr;  // warm-start network output rows [ncol][Nout]
     double* u0out = nullptr;
     Surface sf{};  // AtmosDrivenFluxBC inputs per column (EH)
     // state in other device memory (CSR order) instead of the context's
@@ -564,24 +563,24 @@ void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const dou
     if (!is_eh(c)) {
         hipLaunchKernelGGL((k_rre_coop<G, R, JT>), g, b, 0, c->stream, c->d_col_ptr, bn.d_cols,
                            bn.n, c->d_cpar, c->ncol, c->d_czf, xv, iw, ftw, c->d_cfbw,
-                           rre_bc(c), dt, c->max_iters, c->tol, iters, x.gv, x.u0out);
+                           rre_bc(c), dt, c->max_iters, c->tol, iters, x.wso, x.u0out);
     } else {
         if (is_atmos(c))
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, true>), g, b, 0, c->stream, c->d_col_ptr,
                                bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe, iw, ie,
                                ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
-                               c->tol, iters, x.gv, x.ge, x.u0out, c->bc_top, x.sf);
+                               c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf);
         else if (c->jacobian == CLM_JACOBIAN_NEWTON)
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, false, true>), g, b, 0, c->stream,
                                c->d_col_ptr, bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv,
                                xi, xe, iw, ie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt,
-                               c->max_iters, c->tol, iters, x.gv, x.ge, x.u0out, c->bc_top,
+                               c->max_iters, c->tol, iters, x.wso, x.u0out, c->bc_top,
                                x.sf);
         else
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, false>), g, b, 0, c->stream, c->d_col_ptr,
                                bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe, iw, ie,
                                ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
-                               c->tol, iters, x.gv, x.ge, x.u0out, c->bc_top, x.sf);
+                               c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf);
     }
 }
 
@@ -941,9 +940,10 @@ int ws_features_launch(clm_ctx* c, WsState* w, const double* ftw, const double* 
     return CLM_OK;
 }
 
-// per-cell guesses of the current state into w->gv / w->ge: one fused kernel,
-// or features, the layer-by-layer GEMMs and k_guess (the same arithmetic)
-void ws_guess(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
+// network output rows of the current state into w->Y: one fused kernel
+// (features, four layers), or the features and the layer-by-layer GEMMs (the
+// same arithmetic)
+void ws_infer(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
               bool fused) {
     if (fused) {
         const bf16* W = w->wpk;
@@ -955,16 +955,23 @@ void ws_guess(clm_ctx* c, WsState* w, const double* ftw, const double* fth, doub
                                c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw,
                                fth ? fth : ftw, dt,
                                W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
-                               bias(0), bias(1), bias(2), bias(3), w->gv, w->ge);
+                               bias(0), bias(1), bias(2), bias(3), w->Y);
         else
             hipLaunchKernelGGL(ws::k_ws_infer<false>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
                                c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
                                W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
-                               bias(0), bias(1), bias(2), bias(3), w->gv, w->ge);
+                               bias(0), bias(1), bias(2), bias(3), w->Y);
         return;
     }
     ws_features_launch(c, w, ftw, fth, dt, w->X);
     ws_forward(c, w, w->X, (int)c->ncol, false, w->Y);
+}
+
+// per-cell guesses of the current state into w->gv / w->ge (the step kernels
+// form the same guesses from w->Y themselves)
+void ws_guess(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
+              bool fused) {
+    ws_infer(c, w, ftw, fth, dt, fused);
     hipLaunchKernelGGL(ws::k_guess, grid1(c->ncol * ws::kGroup), dim3(256), 0, c->stream, is_eh(c),
                        c->d_col_ptr, c->ncol, c->d_cpar, c->d_cv, c->d_ci, w->Y, w->d.Nout,
                        w->gv, w->ge);
@@ -974,9 +981,8 @@ void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt
                      StepExtra* x) {
     WsState* w = c->ws;
     if (!w || !w->enabled) return;
-    ws_guess(c, w, ftw, fth, dt, true);
-    x->gv = w->gv;
-    x->ge = is_eh(c) ? w->ge : nullptr;
+    ws_infer(c, w, ftw, fth, dt, true);
+    x->wso = w->Y;
 }
 
 int upload_raw(clm_ctx* c, const double* h, double* d, int64_t n) {
@@ -2352,6 +2358,9 @@ int clm_ws_train(clm_ctx* c, int32_t epochs, int32_t batch, float lr, uint64_t s
     };
     const int Fp = w->d.Fp, No = w->d.Nout;
     for (int32_t ep = 0; ep < epochs; ++ep) {
+        // cosine decay of the step size over the epochs
+        const float lr_ep =
+            lr * 0.5f * (1.0f + (float)std::cos(3.141592653589793 * ep / (double)epochs));
         for (int64_t i = w->rows - 1; i > 0; --i) std::swap(perm[i], perm[next() % (i + 1)]);
         HIPCHK(c, hipMemcpyAsync(w->idx, perm.data(), 4 * perm.size(), hipMemcpyHostToDevice,
                                  c->stream));
@@ -2364,7 +2373,7 @@ int clm_ws_train(clm_ctx* c, int32_t epochs, int32_t batch, float lr, uint64_t s
             hipLaunchKernelGGL(ws::k_mse, grid1((int64_t)batch * No), dim3(256), 0, c->stream,
                                batch, No, w->Yb, w->Tb, w->Mk, w->dY, w->loss);
             ws_backward(c, w, batch);
-            ws_adam(c, w, lr);
+            ws_adam(c, w, lr_ep);
         }
         HIPCHK(c, hipGetLastError());
         float l = 0.0f;

@@ -366,7 +366,8 @@ int clm_ws_enable(clm_ctx* ctx, int32_t on);
 int clm_ws_collect(clm_ctx* ctx, double dt, int32_t nsteps, int32_t steps_per_slot,
                    int32_t stride, int64_t* rows);
 /* Adam on the collected rows (masked MSE), minibatch `batch` (multiple of
- * 512), one loss value per epoch. */
+ * 512), step size lr decayed over the epochs as lr (1 + cos(π ep/epochs))/2,
+ * one loss value per epoch. */
 int clm_ws_train(clm_ctx* ctx, int32_t epochs, int32_t batch, float lr, uint64_t seed,
                  float* loss_per_epoch);
 /* Checking hooks: features of the device state ([ncol x Fp], with the
