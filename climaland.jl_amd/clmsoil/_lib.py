@@ -35,6 +35,7 @@ EXPORTED = (
     "clm_exp_tendency", "clm_imp_tendency", "clm_jacobian", "clm_solve",
     "clm_update_implicit_cache", "clm_get_aux", "clm_get_jacobian", "clm_step", "clm_run",
     "clm_step_noalloc", "clm_totals", "clm_last_kernel_ms", "clm_get_shapes", "clm_set_shape", "clm_autotune",
+    "clm_set_pairing", "clm_get_pairing",
     "clm_ws_init", "clm_ws_dims", "clm_ws_get_params", "clm_ws_set_params", "clm_ws_enable",
     "clm_ws_collect", "clm_ws_train", "clm_ws_features", "clm_ws_forward", "clm_ws_gradient",
     "clm_ws_time", "clm_ws_guess", "clm_save_checkpoint", "clm_read_checkpoint",
@@ -118,6 +119,8 @@ def load(path: str | None = None) -> C.CDLL:
         "clm_get_jacobian": ([vp, _dp, _dp, _dp], C.c_int),
         "clm_step": ([vp, C.c_double, C.c_double, C.POINTER(StepStats)], C.c_int),
         "clm_step_noalloc": ([vp, C.c_double, C.c_double, C.POINTER(StepStats)], C.c_int),
+        "clm_set_pairing": ([vp, C.c_int32], C.c_int),
+        "clm_get_pairing": ([vp, _ip], C.c_int),
         "clm_run": ([vp, C.c_double, C.c_double, C.c_int32, C.c_int32, C.POINTER(StepStats),
                      _ip], C.c_int),
         "clm_totals": ([vp, _dp, _dp], C.c_int),

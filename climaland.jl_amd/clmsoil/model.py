@@ -356,6 +356,15 @@ class _SoilModel:
     def set_shape(self, bucket_max_layers: int, G: int, R: int):
         self._check(self._lib.clm_set_shape(self._ctx, bucket_max_layers, G, R))
 
+    def set_pairing(self, on: bool):
+        """Paired waves for the <= 40 and <= 24 layer buckets (on by default)."""
+        self._check(self._lib.clm_set_pairing(self._ctx, 1 if on else 0))
+
+    def pairing_active(self) -> bool:
+        a = C.c_int32(0)
+        self._check(self._lib.clm_get_pairing(self._ctx, C.byref(a)))
+        return bool(a.value)
+
     def autotune(self, dt: float, reps: int = 2):
         """Time every fitting shape per depth bucket on one step from the
         current device state (pushed by LandSimulation) and keep the fastest."""

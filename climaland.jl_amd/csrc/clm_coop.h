@@ -96,6 +96,82 @@ __device__ __forceinline__ double g_err(double e, double tol) {
     return H < th ? 0.0 : tol;
 }
 
+// A paired wave holds two columns: lanes [0, kPairSplit) one of up to
+// kPairSplit rows, lanes [kPairSplit, 64) one of up to 64 - kPairSplit rows.
+constexpr int kPairSplit = 40;
+
+// The reductions and broadcasts of one column's lanes: a group of G lanes,
+// or (PAIR) the column's part of a paired wave.  The paired reductions work
+// within aligned 8-lane half-rows (DPP), none of which holds lanes of both
+// columns (kPairSplit is a multiple of 8), then take the half-rows' results
+// with lane reads: each column reads only its own lanes, so they stay right
+// while the other column's lanes have left a loop.
+template <int G, bool PAIR>
+struct Grp {
+    bool inA = true;  // PAIR: a lane of the first column
+    static_assert(!PAIR || (G == 64 && kPairSplit % 8 == 0), "paired wave layout");
+    __device__ __forceinline__ uint32_t umax(uint32_t v) const {
+        if constexpr (!PAIR) {
+            return g_umax<G>(v);
+        } else {
+            v = g_umax<8>(v);
+            uint32_t a = 0, b = 0;
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)v, 8 * h + 7);
+                if (8 * h < kPairSplit)
+                    a = max(a, r);
+                else
+                    b = max(b, r);
+            }
+            return inA ? a : b;
+        }
+    }
+    __device__ __forceinline__ double dmax(double v) const {
+        if constexpr (!PAIR) {
+            return g_max<G>(v);
+        } else {
+#pragma unroll
+            for (int o = 4; o > 0; o >>= 1) v = nan_max(v, __shfl_xor(v, o, 8));
+            const long long q = __double_as_longlong(v);
+            double a = 0.0, b = 0.0;
+#pragma unroll
+            for (int h = 0; h < 8; ++h) {
+                const int lo = __builtin_amdgcn_readlane((int)(uint32_t)q, 8 * h + 7);
+                const int hi = __builtin_amdgcn_readlane((int)(uint32_t)(q >> 32), 8 * h + 7);
+                const double r = __longlong_as_double(
+                    (long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+                if (8 * h < kPairSplit)
+                    a = nan_max(a, r);
+                else
+                    b = nan_max(b, r);
+            }
+            return inA ? a : b;
+        }
+    }
+    template <typename T>
+    __device__ __forceinline__ T bcast(T v, int src) const {
+        if constexpr (!PAIR)
+            return g_bcast<G>(v, src);
+        else
+            return __shfl(v, (inA ? 0 : kPairSplit) + src, 64);
+    }
+    // g_err for the column's lanes
+    __device__ __forceinline__ double err(double e, double tol) const {
+        if constexpr (!PAIR) {
+            return g_err<G>(e, tol);
+        } else {
+            const uint32_t h = (uint32_t)((uint64_t)__double_as_longlong(e) >> 32) & 0x7fffffffu;
+            const uint32_t th =
+                (uint32_t)((uint64_t)__double_as_longlong(tol) >> 32) & 0x7fffffffu;
+            const uint32_t H = umax(h);
+            if (H >= 0x7ff00000u) return NAN;
+            if (H == th) return dmax(e);
+            return H < th ? 0.0 : tol;
+        }
+    }
+};
+
 // Reciprocal of a pivot: exact division in FP64; in the FP32 solve of the
 // mixed form (an inexact Newton whose FP64 residual decides convergence) the
 // hardware reciprocal (1 ulp), one instruction instead of the ten of an IEEE
@@ -137,18 +213,34 @@ struct alignas(4 * sizeof(JT)) Q4 {
     JT x, y, z, w;
 };
 
-template <int G, typename JT>
+// A paired wave (PAIR) holds two columns (see kPairSplit).  Its slice is
+// [32 identity][first column][32 identity][second column][32 identity]:
+// reads at distance <= 32 (the six levels of a 64-lane solve) never leave a
+// column's own entries and padding.
+
+template <int G, typename JT, bool PAIR = false>
 struct Xch {
     static_assert(G >= 2 && G <= 64 && (G & (G - 1)) == 0, "group of 2..64 lanes");
-    static constexpr int kWaveEntries = 128;  // (64 / G) groups x 2G entries
-    Q4<JT>* e;                                // this lane's entry
+    static_assert(!PAIR || G == 64, "a paired layout spans the wave");
+    // (64 / G) groups x 2G entries; paired: 3 x 32 padding + 64 lanes
+    static constexpr int kWaveEntries = PAIR ? 160 : 128;
+    Q4<JT>* e;  // this lane's entry
     int gl;
     __device__ __forceinline__ Xch(Q4<JT>* block_slices, int gl_) : gl(gl_) {
         const int lane = threadIdx.x & 63;
-        Q4<JT>* seg = block_slices + (threadIdx.x >> 6) * kWaveEntries + (lane / G) * (2 * G);
-        e = seg + G / 2 + gl;
-        Q4<JT>* pad = (gl < G / 2) ? seg + gl : seg + G + gl;
-        *pad = Q4<JT>{(JT)0, (JT)0, (JT)0, (JT)1};
+        const Q4<JT> id{(JT)0, (JT)0, (JT)0, (JT)1};
+        if constexpr (PAIR) {
+            Q4<JT>* seg = block_slices + (threadIdx.x >> 6) * kWaveEntries;
+            e = seg + (lane < kPairSplit ? 32 + lane : 104 + (lane - kPairSplit));
+            seg[lane < 32 ? lane : 72 + (lane - 32)] = id;  // [0, 32) and [72, 104)
+            if (lane < 32) seg[128 + lane] = id;             // [128, 160)
+        } else {
+            Q4<JT>* seg =
+                block_slices + (threadIdx.x >> 6) * kWaveEntries + (lane / G) * (2 * G);
+            e = seg + G / 2 + gl;
+            Q4<JT>* pad = (gl < G / 2) ? seg + gl : seg + G + gl;
+            *pad = id;
+        }
     }
     __device__ __forceinline__ void put(JT x, JT y, JT z, JT w) {
         __builtin_amdgcn_wave_barrier();
@@ -167,8 +259,8 @@ struct Xch {
 
 // Parallel cyclic reduction, one row per lane (rows >= N are identity rows
 // a = c = d = 0, b = 1).  Returns x with a x_{l-1} + b x_l + c x_{l+1} = d.
-template <int G, typename JT>
-__device__ __forceinline__ JT pcr1(Xch<G, JT>& xc, JT a, JT b, JT c, JT d) {
+template <int G, typename JT, typename XC>
+__device__ __forceinline__ JT pcr1(XC& xc, JT a, JT b, JT c, JT d) {
 #pragma unroll
     for (int s = 1; s < G; s <<= 1) {
         xc.put(a, c, d, recip(b));
@@ -191,8 +283,8 @@ __device__ __forceinline__ JT pcr1(Xch<G, JT>& xc, JT a, JT b, JT c, JT d) {
 // even rows are recovered by back substitution.  The next lane's even row
 // (a, c, d, b) comes through an exchange entry, whose identity padding
 // (b in the fourth slot) is again the row past the column's top.
-template <int G, typename JT>
-__device__ __forceinline__ void pcr2(Xch<G, JT>& xc, const JT* a, const JT* b, const JT* c,
+template <int G, typename JT, typename XC>
+__device__ __forceinline__ void pcr2(XC& xc, const JT* a, const JT* b, const JT* c,
                                      const JT* d, JT* x) {
     xc.put(a[0], c[0], d[0], b[0]);
     const Q4<JT> n = xc.at(1);  // row 2l+2
@@ -232,7 +324,8 @@ struct Pcr1Factor {
     static constexpr int L = Log2<G>::v;
     JT al[L], ga[L];
     JT ib;
-    __device__ __forceinline__ void factor(Xch<G, JT>& xc, JT a, JT b, JT c) {
+    template <typename XC>
+    __device__ __forceinline__ void factor(XC& xc, JT a, JT b, JT c) {
         int lev = 0;
 #pragma unroll
         for (int s = 1; s < G; s <<= 1, ++lev) {
@@ -247,7 +340,8 @@ struct Pcr1Factor {
         }
         ib = (JT)1 / b;
     }
-    __device__ __forceinline__ JT solve(Xch<G, JT>& xc, JT d) const {
+    template <typename XC>
+    __device__ __forceinline__ JT solve(XC& xc, JT d) const {
         int lev = 0;
 #pragma unroll
         for (int s = 1; s < G; s <<= 1, ++lev) {
@@ -265,8 +359,8 @@ template <int G, typename JT>
 struct Pcr2Factor {
     JT al, ga, a0, c0, ib0;
     Pcr1Factor<G, JT> odd;
-    __device__ __forceinline__ void factor(Xch<G, JT>& xc, const JT* a, const JT* b,
-                                           const JT* c) {
+    template <typename XC>
+    __device__ __forceinline__ void factor(XC& xc, const JT* a, const JT* b, const JT* c) {
         xc.put(a[0], c[0], (JT)0, b[0]);
         const Q4<JT> n = xc.at(1);
         ib0 = (JT)1 / b[0];
@@ -276,7 +370,8 @@ struct Pcr2Factor {
         c0 = c[0];
         odd.factor(xc, al * a[0], b[1] + al * c[0] + ga * n.x, ga * n.y);
     }
-    __device__ __forceinline__ void solve(Xch<G, JT>& xc, const JT* d, JT* x) const {
+    template <typename XC>
+    __device__ __forceinline__ void solve(XC& xc, const JT* d, JT* x) const {
         xc.template put1<2>(d[0]);
         const JT d2 = xc.at(1).z;
         JT xo = odd.solve(xc, d[1] + al * d[0] + ga * d2);
@@ -295,8 +390,8 @@ struct Pcr2Factor {
 // (.., x_{R-1} of lane l-1, x_0, x_{R-1}, x_0 of lane l+1, ..): the Schur
 // complement of a diagonally dominant matrix, solved by pcr2.  Interior
 // unknowns follow from the formula above.
-template <int G, int R, typename JT>
-__device__ __forceinline__ void pcr_solve(Xch<G, JT>& xc, JT (&a)[R], JT (&b)[R], JT (&c)[R],
+template <int G, int R, typename JT, typename XC>
+__device__ __forceinline__ void pcr_solve(XC& xc, JT (&a)[R], JT (&b)[R], JT (&c)[R],
                                           JT (&d)[R], JT (&x)[R]) {
     if constexpr (R == 1) {
         x[0] = pcr1<G, JT>(xc, a[0], b[0], c[0], d[0]);
@@ -591,7 +686,16 @@ constexpr int kEhMinBlocks = (std::is_same<JT, float>::value && R == 1) ? 4 : 1;
 // with every iterate, so it is solved in full each iteration; W_ρϑ has no
 // "- I".  A column whose Newton iterate turns non-finite is stepped again from
 // U0 with the reference's matrices (flag bit 0x80000).
-template <int G, int R, typename JT, bool ATM = false, bool NEWTON = false>
+//
+// PAIR (G = 64, R = 1, prescribed-flux top): wave w holds column bin_cols[w]
+// (up to kPairSplit layers) in lanes [0, kPairSplit) and column bin_cols_b[w]
+// (up to 64 - kPairSplit layers; none for w >= ncols_b) in the other lanes,
+// so a column of 33-40 layers no longer leaves a third of its wave idle.
+// Each column's rows see only its own rows and identity padding (Xch), its
+// own lanes' reductions and broadcasts (Grp), and neighbour values from the
+// other column only where a boundary select discards them, so its results
+// are those of a wave of its own, bit for bit.
+template <int G, int R, typename JT, bool ATM = false, bool NEWTON = false, bool PAIR = false>
 __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     const int32_t* __restrict__ col_ptr, const int32_t* __restrict__ bin_cols, int ncols_bin,
     const double* __restrict__ par, int64_t ncol, const double* __restrict__ zf,
@@ -600,17 +704,34 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     const double* __restrict__ fth, const double* __restrict__ fbw,
     const double* __restrict__ fbh, int bc_bottom, double dt, int max_iters, double tol,
     int32_t* __restrict__ iters, const float* __restrict__ wso,
-    double* __restrict__ u0out, int bc_top, Surface sf) {
-    GroupId id = group_id<G>(ncols_bin);
-    if (!id.live) return;
-    const int gl = id.gl;
-    __shared__ Q4<JT> xmem[4 * Xch<G, JT>::kWaveEntries];  // 256 threads: 4 waves
-    Xch<G, JT> xc(xmem, gl);
-    // a 64-lane group is the whole wave: its column index, offsets and
-    // parameters are wave-uniform (scalar registers and loads)
-    const int col = G == 64 ? __builtin_amdgcn_readfirstlane(bin_cols[id.grp]) : bin_cols[id.grp];
+    double* __restrict__ u0out, int bc_top, Surface sf,
+    const int32_t* __restrict__ bin_cols_b, int ncols_b) {
+    static_assert(!PAIR || (G == 64 && R == 1 && !ATM), "paired waves: 64 lanes, one row each");
+    int gl, wave = 0;
+    Grp<G, PAIR> grp;
+    if constexpr (PAIR) {
+        wave = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+        if (wave >= ncols_bin) return;  // wave-uniform
+        const int lane = threadIdx.x & 63;
+        grp.inA = lane < kPairSplit;
+        gl = grp.inA ? lane : lane - kPairSplit;
+    } else {
+        GroupId id = group_id<G>(ncols_bin);
+        if (!id.live) return;
+        gl = id.gl;
+        wave = id.grp;
+    }
+    __shared__ Q4<JT> xmem[4 * Xch<G, JT, PAIR>::kWaveEntries];  // 256 threads: 4 waves
+    Xch<G, JT, PAIR> xc(xmem, gl);
+    // PAIR: the lanes of a missing second column run on the first one's index
+    // with no rows.  A 64-lane group is the whole wave: its column index,
+    // offsets and parameters are wave-uniform (scalar registers and loads).
+    const bool has_col = !PAIR || grp.inA || wave < ncols_b;
+    const int col = PAIR ? ((grp.inA || !has_col) ? bin_cols[wave] : bin_cols_b[wave])
+                         : (G == 64 ? __builtin_amdgcn_readfirstlane(bin_cols[wave])
+                                    : bin_cols[wave]);
     const int64_t base = col_ptr[col];
-    const int N = (int)(col_ptr[col + 1] - base);
+    const int N = has_col ? (int)(col_ptr[col + 1] - base) : 0;
     const int64_t fbase = base + col;
     const EHPar e = load_eh(par, ncol, col);
     double topw = ATM ? 0.0 : ftw[col], toph = ATM ? 0.0 : fth[col];
@@ -728,7 +849,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
     for (int k = 0; k < R; ++k) idzf[k] = dzf[k] != 0.0 ? 1.0 / dzf[k] : 0.0;
     double botw, both;
     {
-        double K0 = g_bcast<G>(Kl[0], 0), T0 = g_bcast<G>(Tp[0], 0);
+        double K0 = grp.bcast(Kl[0], 0), T0 = grp.bcast(Tp[0], 0);
         if (bc_bottom == BC_FREE_DRAINAGE) {  // boundary_conditions.jl:590-608
             botw = -1.0 * K0;
             both = -1.0 * K0 * energy_liq(T0);
@@ -995,14 +1116,14 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             Pp[k] = pd.psi;
             iD[k] = pd.invD;
         }
-        err = g_err<G>(e2, tol);
+        err = grp.err(e2, tol);
         conv = (tol > 0.0) && (err < tol);
         if (conv || !isfinite(err)) break;
     }
     if (kit > max_iters) kit = max_iters;
     kit_total += kit;
     {
-        const bool any_stl = g_umax<G>((uint32_t)stl) != 0u;  // every lane of the group
+        const bool any_stl = grp.umax((uint32_t)stl) != 0u;  // every lane of the group
         stalled = conv && any_stl;
     }
     }
@@ -1015,8 +1136,8 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             xeg[i] = xe[k];
         }
     }
-    const uint32_t bad_g = g_umax<G>((uint32_t)bad);
-    if (gl == 0) {
+    const uint32_t bad_g = grp.umax((uint32_t)bad);
+    if (gl == 0 && has_col) {
         // ∫F: U0 (explicit: sublimation) then the -I block with the step's fluxes
         double iw0 = ATM ? iw[col] + dt * (0.0 + -vice) : iw[col];
         iw[col] = iw0 + dt * (-(topw - botw) - rss);

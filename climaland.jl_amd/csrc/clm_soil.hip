@@ -118,6 +118,8 @@ struct clm_ctx {
     double g_dt = 0.0;
     double last_ms = 0.0;
     int32_t last_launches = 0;
+    // paired waves for the bins of <= 40 and <= 24 layers (launch_pair)
+    bool pairing = true;
     // device address of this GPU's static scratch (clm_step_noalloc)
     unsigned char* d_scratch = nullptr;
     std::string err;
@@ -569,18 +571,18 @@ void launch_bin(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const dou
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, true>), g, b, 0, c->stream, c->d_col_ptr,
                                bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe, iw, ie,
                                ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
-                               c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf);
+                               c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf, nullptr, 0);
         else if (c->jacobian == CLM_JACOBIAN_NEWTON)
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, false, true>), g, b, 0, c->stream,
                                c->d_col_ptr, bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv,
                                xi, xe, iw, ie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt,
                                c->max_iters, c->tol, iters, x.wso, x.u0out, c->bc_top,
-                               x.sf);
+                               x.sf, nullptr, 0);
         else
             hipLaunchKernelGGL((k_eh_coop<G, R, JT, false>), g, b, 0, c->stream, c->d_col_ptr,
                                bn.d_cols, bn.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe, iw, ie,
                                ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
-                               c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf);
+                               c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf, nullptr, 0);
     }
 }
 
@@ -601,10 +603,63 @@ void launch_bin_any(clm_ctx* c, const clm_ctx::Bin& bn, const double* ftw, const
     if (bn.n > 0) launch_menu<JT, 0>(c, menu_index(bn.G, bn.R), bn, ftw, fth, dt, iters, x);
 }
 
+// Paired waves (k_eh_coop<64, 1, JT, false, NEWTON, true>): the bin of 33-40
+// layers on shape (64, 1) and the bin of 17-24 layers on shape (32, 1) share
+// waves, the i-th column of each list in wave i (both lists are ordered by
+// the last step's Newton counts, so a wave's columns iterate alike); the
+// shallow bin's columns beyond the deep bin's count run on their own shape.
+// EnergyHydrology with a prescribed-flux top boundary.
+constexpr int kPairDeepBin = 4, kPairShallowBin = 2;  // buckets of <= 40 and <= 24 layers
+
+bool pairing_applies(const clm_ctx* c) {
+    if (!c->pairing || !is_eh(c) || is_atmos(c) || !is_group(c)) return false;
+    const clm_ctx::Bin &d = c->bins[kPairDeepBin], &h = c->bins[kPairShallowBin];
+    static_assert(kBucketMax[kPairDeepBin] == kPairSplit &&
+                      kBucketMax[kPairShallowBin] <= 64 - kPairSplit,
+                  "paired bins fit the wave's two parts");
+    return d.n > 0 && h.n > 0 && d.G == 64 && d.R == 1 && h.G == 32 && h.R == 1;
+}
+
+template <typename JT>
+void launch_pair(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters,
+                 const StepExtra& x) {
+    const clm_ctx::Bin &d = c->bins[kPairDeepBin], &h = c->bins[kPairShallowBin];
+    const int nb = std::min(d.n, h.n);
+    dim3 g((unsigned)((d.n + 3) / 4)), b(256);  // one wave per deep column
+    double* xv = x.xv ? x.xv : c->d_cv;
+    double* xi = x.xv ? x.xi : c->d_ci;
+    double* xe = x.xv ? x.xe : c->d_ce;
+    double* iw = x.xv ? x.iw : c->d_ciw;
+    double* ie = x.xv ? x.ie : c->d_cie;
+    if (c->jacobian == CLM_JACOBIAN_NEWTON)
+        hipLaunchKernelGGL((k_eh_coop<64, 1, JT, false, true, true>), g, b, 0, c->stream,
+                           c->d_col_ptr, d.d_cols, d.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe,
+                           iw, ie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
+                           c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf, h.d_cols, nb);
+    else
+        hipLaunchKernelGGL((k_eh_coop<64, 1, JT, false, false, true>), g, b, 0, c->stream,
+                           c->d_col_ptr, d.d_cols, d.n, c->d_cpar, c->ncol, c->d_czf, xv, xi, xe,
+                           iw, ie, ftw, fth, c->d_cfbw, c->d_cfbh, c->bc_bottom, dt, c->max_iters,
+                           c->tol, iters, x.wso, x.u0out, c->bc_top, x.sf, h.d_cols, nb);
+    if (h.n > nb) {  // the rest of the shallow bin on its own shape
+        clm_ctx::Bin rest = h;
+        rest.d_cols = h.d_cols + nb;
+        rest.n = h.n - nb;
+        launch_bin_any<JT>(c, rest, ftw, fth, dt, iters, x);
+    }
+}
+
 template <typename JT>
 void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
                     int32_t* iters, const StepExtra& x) {
-    for (const auto& bn : c->bins) launch_bin_any<JT>(c, bn, ftw, fth, dt, iters, x);
+    const bool pair = pairing_applies(c);
+    for (int b = 0; b < (int)c->bins.size(); ++b) {
+        if (pair && b == kPairShallowBin) continue;
+        if (pair && b == kPairDeepBin)
+            launch_pair<JT>(c, ftw, fth, dt, iters, x);
+        else
+            launch_bin_any<JT>(c, c->bins[b], ftw, fth, dt, iters, x);
+    }
 }
 
 // ---- regrouping of a bin's columns by Newton iteration count ---------------------
@@ -949,6 +1004,20 @@ void ws_infer(clm_ctx* c, WsState* w, const double* ftw, const double* fth, doub
         const bf16* W = w->wpk;
         const float* P = w->p;
         auto bias = [&](int l) { return P + w->off[l] + (int64_t)w->ln[l] * w->lk[l]; };
+#ifdef CLM_INFER_V2
+        dim3 g((unsigned)((c->ncol + ws::kIBM2 - 1) / ws::kIBM2)), b(256);
+        if (is_eh(c))
+            hipLaunchKernelGGL(ws::k_ws_infer2<true>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
+                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw,
+                               fth ? fth : ftw, dt,
+                               W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
+                               bias(0), bias(1), bias(2), bias(3), w->Y);
+        else
+            hipLaunchKernelGGL(ws::k_ws_infer2<false>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
+                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
+                               W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
+                               bias(0), bias(1), bias(2), bias(3), w->Y);
+#else
         dim3 g((unsigned)((c->ncol + ws::kIBM - 1) / ws::kIBM)), b(512);
         if (is_eh(c))
             hipLaunchKernelGGL(ws::k_ws_infer<true>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
@@ -961,6 +1030,7 @@ void ws_infer(clm_ctx* c, WsState* w, const double* ftw, const double* fth, doub
                                c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
                                W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
                                bias(0), bias(1), bias(2), bias(3), w->Y);
+#endif
         return;
     }
     ws_features_launch(c, w, ftw, fth, dt, w->X);
@@ -2033,6 +2103,19 @@ int clm_set_shape(clm_ctx* c, int32_t bucket_max_layers, int32_t G, int32_t R) {
 
 constexpr int kTuneSteps = 4;  // consecutive steps per autotune trial
 
+int clm_set_pairing(clm_ctx* c, int32_t on) {
+    if (!c) return CLM_E_ARG;
+    c->pairing = on != 0;
+    destroy_graph(c);
+    return CLM_OK;
+}
+
+int clm_get_pairing(const clm_ctx* c, int32_t* active) {
+    if (!c || !active) return CLM_E_ARG;
+    *active = pairing_applies(c) ? 1 : 0;
+    return CLM_OK;
+}
+
 int clm_autotune(clm_ctx* c, double dt, int32_t reps) {
     if (!c || !(dt > 0.0) || reps < 1) return CLM_E_ARG;
     if (!is_group(c)) return CLM_OK;
@@ -2106,6 +2189,42 @@ int clm_autotune(clm_ctx* c, double dt, int32_t reps) {
         }
         bn.G = bestG;
         bn.R = bestR;
+    }
+    // paired waves (launch_pair) against the per-bucket choice, over whole
+    // steps: the buckets of <= 40 and <= 24 layers on (64, 1) and (32, 1)
+    // sharing waves can beat the shapes each bucket wins on its own
+    if (e == hipSuccess && c->pairing && is_eh(c) && !is_atmos(c) &&
+        c->bins[kPairDeepBin].n > 0 && c->bins[kPairShallowBin].n > 0) {
+        clm_ctx::Bin &d = c->bins[kPairDeepBin], &h = c->bins[kPairShallowBin];
+        const int dG = d.G, dR = d.R, hG = h.G, hR = h.R;
+        auto time_steps = [&](float* out) -> hipError_t {
+            float tmin = 1e30f;
+            hipError_t r2 = hipSuccess;
+            for (int k = 0; k < reps && r2 == hipSuccess; ++k) {
+                r2 = copy_all(false);
+                if (r2 == hipSuccess) r2 = hipEventRecord(c->ev0, c->stream);
+                for (int st = 0; st < kTuneSteps && r2 == hipSuccess; ++st) {
+                    launch_step(c, c->d_cftw, c->d_cfth, dt, c->d_iters);
+                    r2 = hipGetLastError();
+                }
+                if (r2 == hipSuccess) r2 = hipEventRecord(c->ev1, c->stream);
+                if (r2 == hipSuccess) r2 = hipEventSynchronize(c->ev1);
+                float ms = 0.f;
+                if (r2 == hipSuccess) r2 = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+                tmin = std::min(tmin, ms);
+            }
+            *out = tmin;
+            return r2;
+        };
+        float t_sep = 1e30f, t_pair = 1e30f;
+        if (!(dG == 64 && dR == 1 && hG == 32 && hR == 1)) {
+            c->pairing = false;
+            e = time_steps(&t_sep);
+            c->pairing = true;
+            d.G = 64, d.R = 1, h.G = 32, h.R = 1;
+            if (e == hipSuccess) e = time_steps(&t_pair);
+            if (!(t_pair < t_sep)) d.G = dG, d.R = dR, h.G = hG, h.R = hR;
+        }
     }
     if (e == hipSuccess) e = copy_all(false);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);

@@ -314,6 +314,14 @@ int clm_get_shapes(clm_ctx* ctx, int32_t* records /* 4 x max_records */,
                    int32_t max_records, int32_t* nrecords);
 /* Force a bucket's shape; (G, R) must be one of the compiled shapes. */
 int clm_set_shape(clm_ctx* ctx, int32_t bucket_max_layers, int32_t G, int32_t R);
+/* Paired waves (no counterpart in the reference; a launch layout): with the
+ * buckets of <= 40 and <= 24 layers on shapes (64, 1) and (32, 1), an
+ * EnergyHydrology step with a prescribed-flux top boundary runs one column
+ * of each in a wave (40 + 24 lanes) instead of leaving lanes idle.  Results
+ * are those of the separate launches, bit for bit.  On by default;
+ * clm_get_pairing reports whether the current shapes use it. */
+int clm_set_pairing(clm_ctx* ctx, int32_t on);
+int clm_get_pairing(const clm_ctx* ctx, int32_t* active);
 /* Autotuner: for every non-empty bucket, time each compiled shape that fits
  * (best of reps trials, each four consecutive steps from the current state
  * and clm_set_forcing values, regrouped by Newton count between steps; the
