@@ -494,19 +494,21 @@ __global__ __launch_bounds__(512) void k_gemm_wide(
     }
 }
 
-// ---- fused inference: features -> 4 layers in one kernel -----------------------
-// A 512-thread block takes 128 columns.  Their feature rows are built in LDS
-// (the same lanes and arithmetic as k_features), pass through the four layers
-// with the activations kept in LDS (one [128][H + 8] BF16 buffer, overwritten
-// by each hidden layer's output once every wave has finished reading it), and
-// the output layer writes the FP32 output rows [ncol][Nout]; the step kernels
-// turn a column's row into its per-cell guesses as k_guess does.  HBM sees the
-// state, the parameters and the output rows only.  Weights stream from L2 in
-// chunk-major order ([K/32][N][32], written by k_cast_weights / k_adam) through
-// a double-buffered [2][N][32 + 8] LDS buffer, one barrier per 32-wide K chunk.
-// Each layer keeps the accumulation order of k_gemm_wide / k_gemm_nt (K chunks
-// of 32 ascending, two 16-deep MFMA steps per chunk, the same transposed
-// operand order), so the outputs equal the unfused path's bit for bit.
+// ---- fused inference: the four layers in one kernel -----------------------------
+// A 512-thread block takes 128 columns: their feature rows (from k_features,
+// a launch of its own at full occupancy -- inside this kernel, whose LDS
+// admits one block per CU, the features' dependent loads waited with eight
+// waves per CU) are copied into LDS, pass through the four layers with the
+// activations kept in LDS (one [128][H + 8] BF16 buffer, overwritten by each
+// hidden layer's output once every wave has finished reading it), and the
+// output layer writes the FP32 output rows [ncol][Nout]; the step kernels
+// turn a column's row into its per-cell guesses as k_guess does.  Weights
+// stream from L2 in chunk-major order ([K/32][N][32], written by
+// k_cast_weights / k_adam) through a double-buffered [2][N][32 + 8] LDS
+// buffer, one barrier per 32-wide K chunk.  Each layer keeps the
+// accumulation order of k_gemm_wide / k_gemm_nt (K chunks of 32 ascending,
+// two 16-deep MFMA steps per chunk, the same transposed operand order), so
+// the outputs equal the unfused path's bit for bit.
 constexpr int kIBM = 128;
 
 template <bool EH>
@@ -637,33 +639,29 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
     __syncthreads();
 }
 
-// Wpk[l]: chunk-major BF16 weights of layer l; bias[l]: its FP32 bias
+// X: feature rows [ncol][Fp] (k_features); Wpk[l]: chunk-major BF16 weights
+// of layer l; bias[l]: its FP32 bias
 template <bool EH>
 __global__ __launch_bounds__(512, 1) void k_ws_infer(
-    const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
-    const bf16* __restrict__ tails, const double* __restrict__ xv, const double* __restrict__ xi,
-    const double* __restrict__ xe, const double* __restrict__ ftw, const double* __restrict__ fth,
-    double dt, const bf16* __restrict__ W0, const bf16* __restrict__ W1, const bf16* __restrict__ W2,
-    const bf16* __restrict__ W3, const float* __restrict__ b0, const float* __restrict__ b1,
-    const float* __restrict__ b2, const float* __restrict__ b3, float* __restrict__ Yg) {
+    int64_t ncol, const bf16* __restrict__ X, const bf16* __restrict__ W0,
+    const bf16* __restrict__ W1, const bf16* __restrict__ W2, const bf16* __restrict__ W3,
+    const float* __restrict__ b0, const float* __restrict__ b1, const float* __restrict__ b2,
+    const float* __restrict__ b3, float* __restrict__ Yg) {
     using D = NetDims<EH>;
     constexpr int LR = D::H + 8;  // LDS row stride of the activations
     __shared__ __attribute__((aligned(16))) bf16 Act[kIBM * LR];
     __shared__ __attribute__((aligned(16))) bf16 Wb[2 * D::H * kLdsRow];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * kIBM;
-    // features: each wave takes two columns per pass, lane j -> node j.  Rows
-    // past the last column repeat its features (their outputs are never
-    // stored), so every load is unconditional and the unrolled passes can
-    // have their loads in flight together.
     {
         Chunk<D::H> first;
-        first.load(W0, 0);  // layer 1's first weight chunk, in flight during the features
-#pragma unroll 4
-        for (int lc = 2 * w + (lane >> 5); lc < kIBM; lc += 16) {
+        first.load(W0, 0);  // layer 1's first weight chunk, in flight during the copy
+        // the block's feature rows; rows past the last column repeat it
+        // (their outputs are never stored)
+        constexpr int RU = D::Fp / 8;  // uint4 per row
+        for (int q = threadIdx.x; q < kIBM * RU; q += 512) {
+            const int lc = q / RU, k = q % RU;
             const int64_t c = c0 + lc < ncol ? c0 + lc : ncol - 1;
-            feature_lane(EH, col_ptr, ncol, par, tails, xv, xi, xe, ftw, fth, dt, D::Fp, c,
-                         lane & 31, Act + lc * LR);
+            *(uint4*)(Act + lc * LR + 8 * k) = *(const uint4*)(X + c * D::Fp + 8 * k);
         }
         first.store(Wb, 0);
     }
@@ -673,158 +671,6 @@ __global__ __launch_bounds__(512, 1) void k_ws_infer(
     infer_layer<D::H, D::H, true, D::H>(Act, LR, Wb, W1, b1, nullptr, 0, W2);
     infer_layer<D::H, D::H, true, D::No>(Act, LR, Wb, W2, b2, nullptr, 0, W3);
     infer_layer<D::No, D::H, false, 0>(Act, LR, Wb, W3, b3, Yg + c0 * D::No, rows, nullptr);
-}
-
-// ---- fused inference, two blocks per CU ----------------------------------------
-// The same features, layers and output rows with 256 threads on 64 columns:
-// activations [64][H + 8] and one weight chunk [H][32 + 8] in LDS (80,896 B
-// for H = 384), so two blocks share a CU and one block's barriers, chunk
-// loads and epilogues overlap the other's MFMAs.  Each wave computes all 64
-// rows of its column tiles (the same LDS bytes per MFMA as the 128-row
-// kernel's 64 x 96 wave tiles).  The chunk for step ch + 1 is loaded into
-// registers while chunk ch is multiplied, then stored into the single LDS
-// chunk between two barriers.  Accumulation order as in infer_layer, so the
-// outputs are those of the layer-by-layer GEMMs bit for bit.
-constexpr int kIBM2 = 64;
-
-template <int N>
-struct Chunk256 {
-    static constexpr int NU = N * kBK / 8;  // uint4 per chunk
-    static constexpr int NB = (NU + 255) / 256;
-    uint4 r[NB];
-    static __device__ __forceinline__ int slot(int h) {
-        int idx = (int)threadIdx.x + 256 * h;
-        return NU % 256 == 0 ? idx : (idx < NU ? idx : NU - 1);
-    }
-    __device__ __forceinline__ void load(const bf16* __restrict__ Wpk, int ch) {
-        const uint4* src = (const uint4*)(Wpk + (int64_t)ch * N * kBK);
-#pragma unroll
-        for (int h = 0; h < NB; ++h) r[h] = src[slot(h)];
-    }
-    __device__ __forceinline__ void store(bf16* __restrict__ Wb) const {
-#pragma unroll
-        for (int h = 0; h < NB; ++h) {
-            int idx = slot(h);
-            *(uint4*)(Wb + (idx >> 2) * kLdsRow + (idx & 3) * 8) = r[h];
-        }
-    }
-};
-
-// one layer over the block's 64 rows (see infer_layer); on entry chunk 0 of W
-// is in Wb behind a barrier, on exit chunk 0 of Wnext is (NN > 0)
-template <int N, int K, bool HIDDEN, int NN>
-__device__ __forceinline__ void infer_layer2(bf16* __restrict__ Act, int lds_row,
-                                             bf16* __restrict__ Wb,
-                                             const bf16* __restrict__ Wpk,
-                                             const float* __restrict__ bias,
-                                             float* __restrict__ Y, int rows,
-                                             const bf16* __restrict__ Wnext) {
-    constexpr int WN = N / 32 < 4 ? N / 32 : 4;     // waves across columns
-    constexpr int WM = 4 / WN < 2 ? 4 / WN : 2;     // waves across the two row tiles
-    constexpr int TM = 2 / WM, TN = N / (32 * WN);  // 32x32 tiles per wave
-    static_assert(K % kBK == 0 && N % 32 == 0 && TN >= 1 && TM >= 1, "layer shape");
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int wm = w / WN, wn = w % WN;
-    const bool active = w < WM * WN;
-    floatx16 acc[TM][TN];
-#pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-        for (int j = 0; j < TN; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    Chunk256<N> nxt;
-    for (int ch = 0; ch < K / kBK; ++ch) {
-        const bool more = ch + 1 < K / kBK;
-        if (more) nxt.load(Wpk, ch + 1);
-        if (active) {
-#pragma unroll
-            for (int ks = 0; ks < kBK; ks += 16) {
-                bf16x8 a[TM], b[TN];
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-                    a[i] = *(const bf16x8*)(Act + (wm * TM * 32 + i * 32 + (lane & 31)) * lds_row +
-                                            ch * kBK + ks + (lane >> 5) * 8);
-#pragma unroll
-                for (int j = 0; j < TN; ++j)
-                    b[j] = *(const bf16x8*)(Wb + (wn * TN * 32 + j * 32 + (lane & 31)) * kLdsRow +
-                                            ks + (lane >> 5) * 8);
-#pragma unroll
-                for (int i = 0; i < TM; ++i)
-#pragma unroll
-                    for (int j = 0; j < TN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j],
-                                                                            0, 0, 0);
-            }
-        }
-        __syncthreads();  // every wave is done with this chunk (and, last time, with Act)
-        if (more) {
-            nxt.store(Wb);
-            __syncthreads();
-        }
-    }
-    if constexpr (NN > 0) {
-        Chunk256<NN> first;
-        first.load(Wnext, 0);  // in flight during the epilogue
-        if constexpr (HIDDEN) first.store(Wb);
-        else static_assert(HIDDEN, "only hidden layers are followed by a layer");
-    }
-    if (active) {
-#pragma unroll
-        for (int i = 0; i < TM; ++i) {
-            const int m = wm * TM * 32 + i * 32 + (lane & 31);
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int n = wn * TN * 32 + j * 32 + 8 * q + 4 * (lane >> 5);
-                    float4 bb = *(const float4*)(bias + n);
-                    float4 v = {acc[i][j][4 * q] + bb.x, acc[i][j][4 * q + 1] + bb.y,
-                                acc[i][j][4 * q + 2] + bb.z, acc[i][j][4 * q + 3] + bb.w};
-                    if constexpr (HIDDEN) {
-                        bf16x4 hv = {(bf16)gelu(v.x), (bf16)gelu(v.y), (bf16)gelu(v.z),
-                                     (bf16)gelu(v.w)};
-                        *(bf16x4*)(Act + m * lds_row + n) = hv;
-                    } else if (m < rows) {
-                        *(float4*)(Y + (int64_t)m * N + n) = v;
-                    }
-                }
-        }
-    }
-    __syncthreads();
-}
-
-template <bool EH>
-__global__ __launch_bounds__(256, 2) void k_ws_infer2(
-    const int32_t* __restrict__ col_ptr, int64_t ncol, const double* __restrict__ par,
-    const bf16* __restrict__ tails, const double* __restrict__ xv, const double* __restrict__ xi,
-    const double* __restrict__ xe, const double* __restrict__ ftw, const double* __restrict__ fth,
-    double dt, const bf16* __restrict__ W0, const bf16* __restrict__ W1, const bf16* __restrict__ W2,
-    const bf16* __restrict__ W3, const float* __restrict__ b0, const float* __restrict__ b1,
-    const float* __restrict__ b2, const float* __restrict__ b3, float* __restrict__ Yg) {
-    using D = NetDims<EH>;
-    constexpr int LR = D::H + 8;  // LDS row stride of the activations
-    __shared__ __attribute__((aligned(16))) bf16 Act[kIBM2 * LR];
-    __shared__ __attribute__((aligned(16))) bf16 Wb[D::H * kLdsRow];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * kIBM2;
-    {
-        Chunk256<D::H> first;
-        first.load(W0, 0);  // layer 1's first weight chunk, in flight during the features
-#pragma unroll 4
-        for (int lc = 2 * w + (lane >> 5); lc < kIBM2; lc += 8) {
-            const int64_t c = c0 + lc < ncol ? c0 + lc : ncol - 1;
-            feature_lane(EH, col_ptr, ncol, par, tails, xv, xi, xe, ftw, fth, dt, D::Fp, c,
-                         lane & 31, Act + lc * LR);
-        }
-        first.store(Wb);
-    }
-    __syncthreads();
-    const int rows = (int)min<int64_t>(kIBM2, ncol - c0);
-    infer_layer2<D::H, D::Fp, true, D::H>(Act, LR, Wb, W0, b0, nullptr, 0, W1);
-    infer_layer2<D::H, D::H, true, D::H>(Act, LR, Wb, W1, b1, nullptr, 0, W2);
-    infer_layer2<D::H, D::H, true, D::No>(Act, LR, Wb, W2, b2, nullptr, 0, W3);
-    infer_layer2<D::No, D::H, false, 0>(Act, LR, Wb, W3, b3, Yg + c0 * D::No, rows, nullptr);
 }
 
 // ---- small training kernels ---------------------------------------------------

@@ -995,45 +995,27 @@ int ws_features_launch(clm_ctx* c, WsState* w, const double* ftw, const double* 
     return CLM_OK;
 }
 
-// network output rows of the current state into w->Y: one fused kernel
-// (features, four layers), or the features and the layer-by-layer GEMMs (the
-// same arithmetic)
+// network output rows of the current state into w->Y: the features
+// (k_features), then the four layers in one fused kernel or as the
+// layer-by-layer GEMMs (the same arithmetic)
 void ws_infer(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
               bool fused) {
+    ws_features_launch(c, w, ftw, fth, dt, w->X);
     if (fused) {
         const bf16* W = w->wpk;
         const float* P = w->p;
         auto bias = [&](int l) { return P + w->off[l] + (int64_t)w->ln[l] * w->lk[l]; };
-#ifdef CLM_INFER_V2
-        dim3 g((unsigned)((c->ncol + ws::kIBM2 - 1) / ws::kIBM2)), b(256);
-        if (is_eh(c))
-            hipLaunchKernelGGL(ws::k_ws_infer2<true>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
-                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw,
-                               fth ? fth : ftw, dt,
-                               W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
-                               bias(0), bias(1), bias(2), bias(3), w->Y);
-        else
-            hipLaunchKernelGGL(ws::k_ws_infer2<false>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
-                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
-                               W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
-                               bias(0), bias(1), bias(2), bias(3), w->Y);
-#else
         dim3 g((unsigned)((c->ncol + ws::kIBM - 1) / ws::kIBM)), b(512);
         if (is_eh(c))
-            hipLaunchKernelGGL(ws::k_ws_infer<true>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
-                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw,
-                               fth ? fth : ftw, dt,
+            hipLaunchKernelGGL(ws::k_ws_infer<true>, g, b, 0, c->stream, c->ncol, w->X,
                                W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
                                bias(0), bias(1), bias(2), bias(3), w->Y);
         else
-            hipLaunchKernelGGL(ws::k_ws_infer<false>, g, b, 0, c->stream, c->d_col_ptr, c->ncol,
-                               c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce, ftw, ftw, dt,
+            hipLaunchKernelGGL(ws::k_ws_infer<false>, g, b, 0, c->stream, c->ncol, w->X,
                                W + w->offw[0], W + w->offw[1], W + w->offw[2], W + w->offw[3],
                                bias(0), bias(1), bias(2), bias(3), w->Y);
-#endif
         return;
     }
-    ws_features_launch(c, w, ftw, fth, dt, w->X);
     ws_forward(c, w, w->X, (int)c->ncol, false, w->Y);
 }
 
