@@ -37,23 +37,25 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per step of the step kernels measured with rocprofv3 (FETCH_SIZE
 # and WRITE_SIZE summed over the bins' launches, FETCH scaled by the factor a
 # plain device copy of known size reads for it, WRITE likewise); raw sums
-# beside them.  Figures, build and commands:
-# profiles/k_eh_coop_eh_262144_uneven.md ("source" says which build).
-MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 1718.7e6, "raw": 1116.7e6,
-                                                      "source": "mixed+learned form, "
-                                                                "modified-Picard build"},
-                    ("eh_262144_uneven", "mixed"): {"scaled": 1469.1e6, "raw": 985.3e6,
-                                                    "source": "modified-Picard build"}}
+# beside them.  Figures, date, build and commands:
+# profiles/k_eh_coop_eh_262144_uneven.md.
+MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 2736.0e6, "raw": 1944.7e6,
+                                                      "source": "rocprofv3 2026-12-03, build "
+                                                                "eeebc0e, 3 early steps"},
+                    ("eh_262144_uneven", "newton"): {"scaled": 2444.4e6, "raw": 1799.0e6,
+                                                     "source": "rocprofv3 2026-12-03, build "
+                                                               "eeebc0e, 3 early steps"}}
 
 # What bounds the fused step: VALU issue.  rocprofv3 counters of the step
-# kernels on 3 early steps of eh_262144_uneven (profiles/k_eh_coop_eh_262144_uneven.md):
-# SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes per step, SQ_INSTS_VALU per step, and
-# the kernels' time per step in the same build's trace.
-MEASURED_VALU = {"eh_262144_uneven": {"fp64_flop_per_step": 4.604e10,
-                                      "valu_instructions_per_step": 1.046e9,
-                                      "kernel_ms_per_step": 2.496,
-                                      "source": "rocprofv3 counters, 3 early steps, mixed "
-                                                "form, modified-Picard build"}}
+# kernels on 3 early steps of eh_262144_uneven, learned form
+# (profiles/k_eh_coop_eh_262144_uneven.md): SQ_INSTS_VALU_FLOPS_FP64 x 64
+# lanes per step, SQ_INSTS_VALU per step, and the kernels' time per step of
+# the same workload in a kernel trace.
+MEASURED_VALU = {"eh_262144_uneven": {"fp64_flop_per_step": 2.2948e10,
+                                      "valu_instructions_per_step": 5.108e8,
+                                      "kernel_ms_per_step": 1.4044,
+                                      "source": "rocprofv3 2026-12-03, build eeebc0e, 3 early "
+                                                "steps, learned form"}}
 FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 CU_COUNT, ENGINE_GHZ = 256, 2.4
 
@@ -350,7 +352,7 @@ def main():
                      traffic.get("scaled"),
                      "traffic_raw": traffic.get("raw"),
                      "traffic_source": traffic.get("source"),
-                     "kernel": kname + " (one launch per depth bin; per step)",
+                     "kernel": kname + " (one launch per depth bin or bin pair; per step)",
                      "kernel_ms_per_step": r["kernel_ms"],
                      "bytes_per_step": bytes_step,
                      "note": "fused step: achieved = compulsory HBM bytes / kernel time; the "
