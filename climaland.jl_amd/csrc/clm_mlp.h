@@ -88,7 +88,15 @@ __global__ __launch_bounds__(256) void k_static_tails(bool eh, const int32_t* __
     o[6] = o[7] = (bf16)0.0f;
 }
 
-// lane j of column c's group writes entries j, 32 + j, ... of the column's row
+// x * (1/y) with the hardware FP32 reciprocal: a few FP32 ulp, far below the
+// BF16 rounding every feature goes through
+__device__ __forceinline__ double fdiv_feat(double x, double y) {
+    return x * (double)__builtin_amdgcn_rcpf((float)y);
+}
+
+// lane j of column c's group writes entries j, 32 + j, ... of the column's
+// row: node j's state features from the two cells around it (each loaded
+// once), then the tail features
 __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict__ col_ptr,
                                              int64_t ncol, const double* __restrict__ par,
                                              const bf16* __restrict__ tails,
@@ -101,16 +109,21 @@ __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict_
     const int64_t b = col_ptr[c];
     const int N = (int)(col_ptr[c + 1] - b);
     const double nu = par[P_NU * ncol + c], thr = par[P_THETA_R * ncol + c];
-    row[j] = feat((at_node(j, N, [&](int i) { return xv[b + i]; }) - thr) / (nu - thr));
+    const NodeAt a = node_at(j, N);
+    const double v0 = xv[b + a.i0], v1 = xv[b + a.i1];
+    row[j] = feat(fdiv_feat(lerp_node(a, v0, v1) - thr, nu - thr));
     int base = kNodes;
     if (eh) {
         const double rcd = par[P_RHOC_DS * ncol + c];
-        double T = at_node(j, N, [&](int i) {
-            double v = xv[b + i], t = xi[b + i];
-            return temperature(xe[b + i], t, heat_capacity(fmin(nu - t, v), t, rcd));
-        });
-        row[kNodes + j] = feat((T - 273.15) / 10.0);
-        row[2 * kNodes + j] = feat(at_node(j, N, [&](int i) { return xi[b + i]; }) / nu);
+        const double t0 = xi[b + a.i0], t1 = xi[b + a.i1];
+        const double e0 = xe[b + a.i0], e1 = xe[b + a.i1];
+        // T - T_0 of each cell (energy_hydrology.jl:430-440 with θ_l = min(ν - θ_i, ϑ_l))
+        const double d0 =
+            fdiv_feat(e0 + t0 * RHO_I * LH_F0, heat_capacity(fmin(nu - t0, v0), t0, rcd));
+        const double d1 =
+            fdiv_feat(e1 + t1 * RHO_I * LH_F0, heat_capacity(fmin(nu - t1, v1), t1, rcd));
+        row[kNodes + j] = feat(((T_0 - 273.15) + lerp_node(a, d0, d1)) * 0.1);
+        row[2 * kNodes + j] = feat(fdiv_feat(lerp_node(a, t0, t1), nu));
         base = 3 * kNodes;
     }
     // tail i in tail_feature's numbering (RichardsModel skips the heat flux)
@@ -124,10 +137,8 @@ __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict_
                 v = tails[c * kTails + i];
             } else if (i == 8) {
                 v = tails[c * kTails + 5];
-            } else {  // forcing and step: one division, the same as tail_feature's
-                double num = i == 5 ? ftw[c] : (i == 6 ? fth[c] : dt);
-                double den = i == 5 ? 1e-5 : (i == 6 ? 100.0 : 3600.0);
-                v = feat(num / den);
+            } else {  // forcing and step, in tail_feature's units
+                v = feat(i == 5 ? ftw[c] * 1e5 : (i == 6 ? fth[c] * 0.01 : dt * (1.0 / 3600.0)));
             }
         }
         row[f] = v;
@@ -231,19 +242,22 @@ __global__ __launch_bounds__(256) void k_guess(
 // product to Cf + z * M * ldcf).
 enum Epi { EPI_F32 = 0, EPI_BIAS_F32 = 1, EPI_BIAS_GELU = 2, EPI_DGELU = 3 };
 
-// GELU, tanh form: z Φ(z) ≈ z σ(2 sqrt(2/π) (z + 0.044715 z^3)), and its
-// derivative; σ from the hardware exp and reciprocal (a few ulp in FP32, far
-// below the BF16 rounding of the result)
-__device__ __forceinline__ float sigm(float u) {
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-u));
+// GELU, tanh form: z Φ(z) ≈ z σ(u), u = 2 sqrt(2/π) (z + 0.044715 z^3), and
+// its derivative.  σ(u) = 1 / (1 + 2^t) with t = -u log2(e) = z (A + B z^2)
+// (the constants folded: two multiplies and an FMA before the hardware
+// exp2 and reciprocal, a few FP32 ulp, far below the BF16 rounding of the
+// result)
+constexpr float kGeluA = -2.302208198144325f;   // -2 sqrt(2/π) / ln 2
+constexpr float kGeluB = -0.1029432395800235f;  // kGeluA * 0.044715
+__device__ __forceinline__ float gelu_sigma(float z, float z2) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(z2, kGeluB, kGeluA)));
 }
-__device__ __forceinline__ float gelu(float z) {
-    return z * sigm(1.5957691216057308f * (z + 0.044715f * z * z * z));
-}
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sigma(z, z * z); }
+// d/dz z σ(u) = σ + z σ (1 - σ) du/dz, du/dz = 2 sqrt(2/π) (1 + 3 0.044715 z^2)
 __device__ __forceinline__ float dgelu(float z) {
-    float z2 = z * z;
-    float sg = sigm(1.5957691216057308f * (z + 0.044715f * z2 * z));
-    return sg + z * sg * (1.0f - sg) * 1.5957691216057308f * (1.0f + 0.134145f * z2);
+    const float z2 = z * z;
+    const float sg = gelu_sigma(z, z2);
+    return sg + z * sg * (1.0f - sg) * fmaf(z2, 0.21406444881780076f, 1.5957691216057308f);
 }
 
 constexpr int kBM = 256, kBN = 128, kBK = 32, kLdsRow = kBK + 8;
