@@ -571,7 +571,7 @@ __global__ __launch_bounds__(256) void k_rre_coop(
         double K[R], h[R], iD[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            PsiI p = psi_invd(v, sp, x[k]);
+            PsiI p = psi_invd<JT>(v, sp, x[k]);
             K[k] = on[k] ? hyd_cond_from_a(v, p.S, p.a) : 0.0;
             h[k] = p.psi + zc[k];
             iD[k] = p.invD;
@@ -745,6 +745,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         dzt = ztop - 0.5 * (zf[fbase + N - 1] + ztop);
     }
 
+    const bool warm = wso != nullptr;
     double xv[R], xi[R], xe[R], u0v[R], u0e[R], zc[R], dzc[R], dzf[R];
     double Kl[R], kl[R], rcs[R], Tp[R], Pp[R], iD[R];
     // Newton form: ρc_s and ∂T/∂ϑ_l at the iterate
@@ -798,8 +799,10 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         Tp[k] = a.T;
         Pp[k] = a.psi;
         sp[k] = make_span(e.v, e.nu - xi[k]);
-        iD[k] = psi_invd(e.v, sp[k], xv[k]).invD;  // Wfact at x = U0
-        if constexpr (NEWTON) newton_dT(e, xv[k], xi[k], Tp[k], rcx[k], dTdv[k]);
+        if (!warm) {  // Wfact at x = U0 (a warm start evaluates it at its own x0)
+            iD[k] = psi_invd<JT>(e.v, sp[k], xv[k]).invD;
+            if constexpr (NEWTON) newton_dT<JT>(e, xv[k], xi[k], Tp[k], rcx[k], dTdv[k]);
+        }
     }
     if constexpr (ATM) {
         // update_infiltration_water_flux! (Runoff.jl:234-290) and the top fluxes
@@ -902,7 +905,6 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         else
             efac.factor(xc, ea, eb, ec);
     }
-    const bool warm = wso != nullptr;
     int kit = 0, kit_total = 0;
     bool conv = false, fell_back = false, stalled = false;
     double err = 0.0;
@@ -924,10 +926,10 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
                                      heat_capacity(fmin(e.nu - t0, v0), t0, e.rhoc_ds);
             }
             if constexpr (NEWTON)
-                newton_T(e, xv[k], xi[k], xe[k], Tp[k], rcx[k], dTdv[k]);
+                newton_T<JT>(e, xv[k], xi[k], xe[k], Tp[k], rcx[k], dTdv[k]);
             else
                 Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
-            PsiI pd = psi_invd(e.v, sp[k], xv[k]);
+            PsiI pd = psi_invd<JT>(e.v, sp[k], xv[k]);
             Pp[k] = pd.psi;
             iD[k] = pd.invD;
         }
@@ -950,7 +952,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             Pp[k] = a.psi;
             xv[k] = u0v[k];
             xe[k] = u0e[k];
-            iD[k] = psi_invd(e.v, sp[k], xv[k]).invD;
+            iD[k] = psi_invd<JT>(e.v, sp[k], xv[k]).invD;
         }
     }
     bool stl = false;  // this lane's rows of the last residual above kStall
@@ -1109,10 +1111,10 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
             // cache_imp!(x): T and ψ at the new iterate (energy_hydrology.jl:427-445);
             // dψ/dϑ of the same evaluation is the next Wfact(x)
             if constexpr (NEWTON)
-                newton_T(e, xv[k], xi[k], xe[k], Tp[k], rcx[k], dTdv[k]);
+                newton_T<JT>(e, xv[k], xi[k], xe[k], Tp[k], rcx[k], dTdv[k]);
             else
                 Tp[k] = eh_imp_T(e, xv[k], xi[k], xe[k]);
-            PsiI pd = psi_invd(e.v, sp[k], xv[k]);
+            PsiI pd = psi_invd<JT>(e.v, sp[k], xv[k]);
             Pp[k] = pd.psi;
             iD[k] = pd.invD;
         }

@@ -72,18 +72,31 @@ __device__ __forceinline__ double eh_imp_T(const EHPar& e, double vl, double ti,
 
 // Newton form (no counterpart in the reference): ρc_s at θ_l = min(ν-θ_i, ϑ_l)
 // of the iterate and ∂T/∂ϑ_l = -(T - T_0) ρ_l c_l / ρc_s where ϑ_l < ν - θ_i
-// (0 above, where θ_l is capped), for a given T ...
+// (0 above, where θ_l is capped), for a given T ...  ∂T/∂ϑ_l enters only the
+// matrix: in Q = float forms (FP32 matrices) through the hardware
+// reciprocal of ρc_s, in the FP64 form through the division
+template <typename Q>
+__device__ __forceinline__ double newton_dTdv(const EHPar& e, double vl, double ti, double T,
+                                              double rc) {
+    if (!(vl < e.nu - ti)) return 0.0;
+    if constexpr (std::is_same<Q, float>::value)
+        return -(T - T_0) * RCP_L * __builtin_amdgcn_rcp(rc);
+    else
+        return -(T - T_0) * RCP_L / rc;
+}
+template <typename Q>
 __device__ __forceinline__ void newton_dT(const EHPar& e, double vl, double ti, double T,
                                           double& rc, double& dTdv) {
     rc = heat_capacity(fmin(e.nu - ti, vl), ti, e.rhoc_ds);
-    dTdv = vl < e.nu - ti ? -(T - T_0) * RCP_L / rc : 0.0;
+    dTdv = newton_dTdv<Q>(e, vl, ti, T, rc);
 }
 // ... and with T of the implicit cache, eh_imp_T's arithmetic
+template <typename Q>
 __device__ __forceinline__ void newton_T(const EHPar& e, double vl, double ti, double re,
                                          double& T, double& rc, double& dTdv) {
     rc = heat_capacity(fmin(e.nu - ti, vl), ti, e.rhoc_ds);
     T = temperature(re, ti, rc);
-    dTdv = vl < e.nu - ti ? -(T - T_0) * RCP_L / rc : 0.0;
+    dTdv = newton_dTdv<Q>(e, vl, ti, T, rc);
 }
 
 // ---- AtmosDrivenFluxBC, soil-only form (boundary_conditions.jl:880-915) --------

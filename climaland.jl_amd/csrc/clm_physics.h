@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 namespace clm {
 
 // ClimaParams 1.1.4 defaults (src/shared_utilities/Parameters.jl:86-105)
@@ -226,6 +228,12 @@ __device__ __forceinline__ Span make_span(const VG& v, double nu_eff) {
     return sp;
 }
 
+// Q: the precision of 1/D.  double: the division, as the FP64 form's
+// Jacobian needs; float (FP32-Jacobian forms, whose matrices are FP32 and
+// whose update only has to be an inexact-Newton step): the hardware FP64
+// reciprocal times the numerator, well inside FP32 accuracy over the whole
+// FP64 range, at a fraction of a division's cost.
+template <typename Q = double>
 __device__ __forceinline__ PsiI psi_invd(const VG& v, const Span& sp, double vl) {
     PsiI r;
     double vs = fmax(vl, v.thr_safe);
@@ -240,7 +248,11 @@ __device__ __forceinline__ PsiI psi_invd(const VG& v, const Span& sp, double vl)
         r.a = a;
         r.psi = -X1n * v.inv_alpha;
         // D is +Inf where X == 0 (within an ulp of S = 1): 1/D = 0
-        r.invD = (X > 0.0) ? (X * S) / (v.cD * sp.inv * X1n * a) : 0.0;
+        const double den = v.cD * sp.inv * X1n * a;
+        if constexpr (std::is_same<Q, float>::value)
+            r.invD = (X > 0.0) ? (X * S) * __builtin_amdgcn_rcp(den) : 0.0;
+        else
+            r.invD = (X > 0.0) ? (X * S) / den : 0.0;
     } else {
         r.a = 1.0;
         r.psi = (S == 1.0) ? -0.0 : (vs - sp.ns) * v.inv_Ss;
