@@ -26,7 +26,6 @@
 namespace clm {
 namespace ws {
 
-typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
@@ -35,16 +34,7 @@ struct Dims {
     int F, Fp, H, Nout;
 };
 __host__ __device__ inline Dims dims_for(bool eh) {
-    return eh ? Dims{105, 128, 384, 64} : Dims{40, 64, 256, 32};
-}
-
-// Features are O(1) by construction.  A column whose iterate blew up enters
-// the network as 0 (non-finite) or clamped to +-20 (finite but absurd), so
-// that no activation overflows: such columns are masked out of training
-// (a zero loss gradient times an overflowed activation would still be NaN)
-// and their guess is discarded when Newton falls back to the plain start.
-__device__ __forceinline__ bf16 feat(double x) {
-    return (bf16)(float)(isfinite(x) ? fmin(fmax(x, -20.0), 20.0) : 0.0);
+    return eh ? Dims{105, kFpEh, 384, 64} : Dims{40, kFpRre, 256, 32};
 }
 
 // The per-column kernels below give each column a group of 32 lanes (lane j
@@ -72,7 +62,6 @@ __device__ __forceinline__ double tail_feature(int i, bool eh, double al, double
 // ---- features of the current state: 32 lanes per column -----------------------
 // Tail features that depend on the column alone (log10 α, n - 2, K_sat, ν,
 // θ_r, N / 40) are rounded once per context into tails[c][0..5]
-constexpr int kTails = 8;
 __global__ __launch_bounds__(256) void k_static_tails(bool eh, const int32_t* __restrict__ col_ptr,
                                                       int64_t ncol, const double* __restrict__ par,
                                                       bf16* __restrict__ tails) {
@@ -86,12 +75,6 @@ __global__ __launch_bounds__(256) void k_static_tails(bool eh, const int32_t* __
                                  par[P_THETA_R * ncol + c], 0.0, 0.0, 0.0, N));
     o[5] = feat((double)N / 40.0);
     o[6] = o[7] = (bf16)0.0f;
-}
-
-// x * (1/y) with the hardware FP32 reciprocal: a few FP32 ulp, far below the
-// BF16 rounding every feature goes through
-__device__ __forceinline__ double fdiv_feat(double x, double y) {
-    return x * (double)__builtin_amdgcn_rcpf((float)y);
 }
 
 // lane j of column c's group writes entries j, 32 + j, ... of the column's
@@ -109,40 +92,19 @@ __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict_
     const int64_t b = col_ptr[c];
     const int N = (int)(col_ptr[c + 1] - b);
     const double nu = par[P_NU * ncol + c], thr = par[P_THETA_R * ncol + c];
-    const NodeAt a = node_at(j, N);
+    const NodeAtF a = node_at_f(j, N);
     const double v0 = xv[b + a.i0], v1 = xv[b + a.i1];
-    row[j] = feat(fdiv_feat(lerp_node(a, v0, v1) - thr, nu - thr));
-    int base = kNodes;
+    row[j] = feat_v(a, (float)thr, (float)nu, (float)v0, (float)v1);
     if (eh) {
         const double rcd = par[P_RHOC_DS * ncol + c];
         const double t0 = xi[b + a.i0], t1 = xi[b + a.i1];
-        const double e0 = xe[b + a.i0], e1 = xe[b + a.i1];
-        // T - T_0 of each cell (energy_hydrology.jl:430-440 with θ_l = min(ν - θ_i, ϑ_l))
-        const double d0 =
-            fdiv_feat(e0 + t0 * RHO_I * LH_F0, heat_capacity(fmin(nu - t0, v0), t0, rcd));
-        const double d1 =
-            fdiv_feat(e1 + t1 * RHO_I * LH_F0, heat_capacity(fmin(nu - t1, v1), t1, rcd));
-        row[kNodes + j] = feat(((T_0 - 273.15) + lerp_node(a, d0, d1)) * 0.1);
-        row[2 * kNodes + j] = feat(fdiv_feat(lerp_node(a, t0, t1), nu));
-        base = 3 * kNodes;
+        row[kNodes + j] = feat_T(a, feat_dT(nu, rcd, v0, t0, xe[b + a.i0]),
+                                 feat_dT(nu, rcd, v1, t1, xe[b + a.i1]));
+        row[2 * kNodes + j] = feat_i(a, (float)nu, (float)t0, (float)t1);
     }
-    // tail i in tail_feature's numbering (RichardsModel skips the heat flux)
-    const int ntail = eh ? 9 : 8;
-    for (int f = base + j; f < Fp; f += kGroup) {
-        int i = f - base;
-        if (!eh && i >= 6) ++i;
-        bf16 v = (bf16)0.0f;
-        if (f - base < ntail) {
-            if (i < 5) {
-                v = tails[c * kTails + i];
-            } else if (i == 8) {
-                v = tails[c * kTails + 5];
-            } else {  // forcing and step, in tail_feature's units
-                v = feat(i == 5 ? ftw[c] * 1e5 : (i == 6 ? fth[c] * 0.01 : dt * (1.0 / 3600.0)));
-            }
-        }
-        row[f] = v;
-    }
+    const int S = state_features(eh);
+    for (int f = S + j; f < Fp; f += kGroup)
+        row[f] = tail_entry(eh, f - S, tails + c * kTails, ftw[c], eh ? fth[c] : 0.0, dt);
 }
 
 __global__ __launch_bounds__(256) void k_features(
@@ -529,20 +491,23 @@ template <bool EH>
 struct NetDims;
 template <>
 struct NetDims<true> {
-    static constexpr int Fp = 128, H = 384, No = 64;
+    static constexpr int Fp = kFpEh, H = 384, No = 64;
 };
 template <>
 struct NetDims<false> {
-    static constexpr int Fp = 64, H = 256, No = 32;
+    static constexpr int Fp = kFpRre, H = 256, No = 32;
 };
 
 // weight chunk ch ([N][32] of the chunk-major layout) -> registers -> LDS
 template <int N>
 struct Chunk {
-    static constexpr int NU = N * kBK / 8;  // uint4 per chunk
+    static constexpr int NU = N * kBK / 8;  // 16-byte pieces per chunk
     static constexpr int NB = (NU + 511) / 512;
-    uint4 r[NB];
-    // threads past the chunk's end repeat its last uint4 (same value, same
+    // a native 4 x u32 vector: copies are plain loads and stores (a uint4
+    // struct copies through memcpy, which left two sets of these in scratch)
+    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+    u32x4 r[NB];
+    // threads past the chunk's end repeat its last piece (same value, same
     // LDS slot): every access is unconditional, so r stays in registers and
     // the loads stay in flight until the store
     static __device__ __forceinline__ int slot(int h) {
@@ -550,7 +515,7 @@ struct Chunk {
         return NU % 512 == 0 ? idx : (idx < NU ? idx : NU - 1);
     }
     __device__ __forceinline__ void load(const bf16* __restrict__ Wpk, int ch) {
-        const uint4* src = (const uint4*)(Wpk + (int64_t)ch * N * kBK);
+        const u32x4* src = (const u32x4*)(Wpk + (int64_t)ch * N * kBK);
 #pragma unroll
         for (int h = 0; h < NB; ++h) r[h] = src[slot(h)];
     }
@@ -558,7 +523,7 @@ struct Chunk {
 #pragma unroll
         for (int h = 0; h < NB; ++h) {
             int idx = slot(h);
-            *(uint4*)(Wb + buf * N * kLdsRow + (idx >> 2) * kLdsRow + (idx & 3) * 8) = r[h];
+            *(u32x4*)(Wb + buf * N * kLdsRow + (idx >> 2) * kLdsRow + (idx & 3) * 8) = r[h];
         }
     }
 };
@@ -591,11 +556,21 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    Chunk<N> nxt;
-    int buf = 0;
-    for (int ch = 0; ch < K / kBK; ++ch) {
-        const bool more = ch + 1 < K / kBK;
-        if (more) nxt.load(Wpk, ch + 1);
+    // Chunk c of W is loaded into register set c % 2 two chunks ahead of its
+    // use (two chunks of MFMAs for the load to arrive in) and stored into LDS
+    // buffer c % 2 behind the barrier that ends the reads of that buffer.
+    // Loads and stores are unconditional (past the last chunk they repeat
+    // it into the free buffer), so the sets stay in registers.
+    constexpr int NCH = K / kBK;
+    Chunk<N> r0, r1;
+    r1.load(Wpk, NCH > 1 ? 1 : 0);
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int nx = ch + 2 < NCH ? ch + 2 : NCH - 1;
+        if (ch % 2 == 0)
+            r0.load(Wpk, nx);
+        else
+            r1.load(Wpk, nx);
         if (active) {
 #pragma unroll
             for (int ks = 0; ks < kBK; ks += 16) {
@@ -606,7 +581,7 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                             ch * kBK + ks + (lane >> 5) * 8);
 #pragma unroll
                 for (int j = 0; j < TN; ++j)
-                    b[j] = *(const bf16x8*)(Wb + buf * N * WS +
+                    b[j] = *(const bf16x8*)(Wb + (ch % 2) * N * WS +
                                             (wn * TN * 32 + j * 32 + (lane & 31)) * WS + ks +
                                             (lane >> 5) * 8);
 #pragma unroll
@@ -617,9 +592,13 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                                                             0, 0, 0);
             }
         }
-        if (more) nxt.store(Wb, buf ^ 1);  // the other buffer was last read before the previous barrier
+        // chunk ch + 1 (in set (ch + 1) % 2) into the other buffer, last read
+        // before the previous barrier
+        if (ch % 2 == 0)
+            r1.store(Wb, 1);
+        else
+            r0.store(Wb, 0);
         __syncthreads();
-        buf ^= 1;
     }
     // every wave is past its last read of Act and Wb
     if constexpr (NN > 0) {
