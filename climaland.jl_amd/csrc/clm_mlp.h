@@ -29,6 +29,10 @@ namespace ws {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+// 16 bytes as a native vector: copies through registers are plain loads and
+// stores (a uint4 struct copies through memcpy, which can leave the staging
+// registers of a tile in scratch)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 struct Dims {
     int F, Fp, H, Nout;
@@ -243,21 +247,21 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    const uint4 zero4 = {0u, 0u, 0u, 0u};
-    uint4 ra[4], rb[2];
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+    u32x4 ra[4], rb[2];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             int cidx = t + 256 * h;
             int row = cidx >> 2, kq = (cidx & 3) * 8;
-            ra[h] = (m0 + row < M) ? *(const uint4*)(A + (int64_t)(m0 + row) * lda + k0 + kq)
+            ra[h] = (m0 + row < M) ? *(const u32x4*)(A + (int64_t)(m0 + row) * lda + k0 + kq)
                                    : zero4;
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             int cidx = t + 256 * h;
             int row = cidx >> 2, kq = (cidx & 3) * 8;
-            rb[h] = (n0 + row < N) ? *(const uint4*)(B + (int64_t)(n0 + row) * ldb + k0 + kq)
+            rb[h] = (n0 + row < N) ? *(const u32x4*)(B + (int64_t)(n0 + row) * ldb + k0 + kq)
                                    : zero4;
         }
     };
@@ -265,12 +269,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm_nt(
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
             int cidx = t + 256 * h;
-            *(uint4*)(As + (cidx >> 2) * kLdsRow + (cidx & 3) * 8) = ra[h];
+            *(u32x4*)(As + (cidx >> 2) * kLdsRow + (cidx & 3) * 8) = ra[h];
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             int cidx = t + 256 * h;
-            *(uint4*)(Bs + (cidx >> 2) * kLdsRow + (cidx & 3) * 8) = rb[h];
+            *(u32x4*)(Bs + (cidx >> 2) * kLdsRow + (cidx & 3) * 8) = rb[h];
         }
     };
     if (kbeg < kend) {
@@ -365,8 +369,8 @@ __global__ __launch_bounds__(512) void k_gemm_wide(
     const float* __restrict__ bias, float* __restrict__ Cf, int ldcf, bf16* __restrict__ Cb,
     int ldcb, const float* __restrict__ Z, int ldz) {
     constexpr int BN = 128 * NTW;
-    constexpr int NA = kWBM * kBK / 8 / 512;  // uint4 of A per thread per chunk (1)
-    constexpr int NB = BN * kBK / 8 / 512;    // uint4 of B per thread per chunk (NTW)
+    constexpr int NA = kWBM * kBK / 8 / 512;  // 16-byte pieces of A per thread per chunk (1)
+    constexpr int NB = BN * kBK / 8 / 512;    // of B per thread per chunk (NTW)
     __shared__ __attribute__((aligned(16))) bf16 As[2][kWBM * kLdsRow];
     __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN * kLdsRow];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -379,33 +383,33 @@ __global__ __launch_bounds__(512) void k_gemm_wide(
         for (int j = 0; j < NTW; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    const uint4 zero4 = {0u, 0u, 0u, 0u};
-    uint4 ra[NA], rb[NB];
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+    u32x4 ra[NA], rb[NB];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int h = 0; h < NA; ++h) {
             int cidx = t + 512 * h;
             int row = cidx >> 2, kq = (cidx & 3) * 8;
-            ra[h] = (m0 + row < M) ? *(const uint4*)(A + (int64_t)(m0 + row) * lda + k0 + kq)
+            ra[h] = (m0 + row < M) ? *(const u32x4*)(A + (int64_t)(m0 + row) * lda + k0 + kq)
                                    : zero4;
         }
 #pragma unroll
         for (int h = 0; h < NB; ++h) {
             int cidx = t + 512 * h;
             int row = cidx >> 2, kq = (cidx & 3) * 8;
-            rb[h] = *(const uint4*)(B + (int64_t)row * ldb + k0 + kq);
+            rb[h] = *(const u32x4*)(B + (int64_t)row * ldb + k0 + kq);
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
         for (int h = 0; h < NA; ++h) {
             int cidx = t + 512 * h;
-            *(uint4*)(&As[buf][(cidx >> 2) * kLdsRow + (cidx & 3) * 8]) = ra[h];
+            *(u32x4*)(&As[buf][(cidx >> 2) * kLdsRow + (cidx & 3) * 8]) = ra[h];
         }
 #pragma unroll
         for (int h = 0; h < NB; ++h) {
             int cidx = t + 512 * h;
-            *(uint4*)(&Bs[buf][(cidx >> 2) * kLdsRow + (cidx & 3) * 8]) = rb[h];
+            *(u32x4*)(&Bs[buf][(cidx >> 2) * kLdsRow + (cidx & 3) * 8]) = rb[h];
         }
     };
     gload(0);
@@ -503,9 +507,6 @@ template <int N>
 struct Chunk {
     static constexpr int NU = N * kBK / 8;  // 16-byte pieces per chunk
     static constexpr int NB = (NU + 511) / 512;
-    // a native 4 x u32 vector: copies are plain loads and stores (a uint4
-    // struct copies through memcpy, which left two sets of these in scratch)
-    typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
     u32x4 r[NB];
     // threads past the chunk's end repeat its last piece (same value, same
     // LDS slot): every access is unconditional, so r stays in registers and
@@ -540,7 +541,10 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                             const bf16* __restrict__ Wpk,
                                             const float* __restrict__ bias,
                                             float* __restrict__ Y, int rows,
-                                            const bf16* __restrict__ Wnext) {
+                                            const bf16* __restrict__ Wnext,
+                                            Chunk<N>& s1,
+                                            Chunk<NN ? NN : 32>& n0,
+                                            Chunk<NN ? NN : 32>& n1) {
     constexpr int WN = N / 32 < 4 ? N / 32 : 4;          // waves across columns
     constexpr int WM = 8 / WN < 4 ? 8 / WN : 4;          // waves across rows
     constexpr int TM = 4 / WM, TN = N / (32 * WN);       // 32x32 tiles per wave
@@ -561,16 +565,28 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
     // buffer c % 2 behind the barrier that ends the reads of that buffer.
     // Loads and stores are unconditional (past the last chunk they repeat
     // it into the free buffer), so the sets stay in registers.
+    // Chunk c of W sits in register set c % 2 (s0 even, s1 odd; s1 arrives
+    // holding chunk 1) from two chunks before its use, so its load has two
+    // chunks of MFMAs to arrive in, and goes into LDS buffer c % 2 behind
+    // the barrier that ends the reads of that buffer.  The last two chunks'
+    // slots load the next layer's chunks 0 and 1 (n0, n1): chunk 0 goes into
+    // buffer 0 once this layer's reads are done, chunk 1 stays in n1.
     constexpr int NCH = K / kBK;
-    Chunk<N> r0, r1;
-    r1.load(Wpk, NCH > 1 ? 1 : 0);
+    static_assert(NCH >= 2, "two chunks at least");
+    Chunk<N> s0;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
-        const int nx = ch + 2 < NCH ? ch + 2 : NCH - 1;
-        if (ch % 2 == 0)
-            r0.load(Wpk, nx);
-        else
-            r1.load(Wpk, nx);
+        if (ch + 2 < NCH) {
+            if (ch % 2 == 0)
+                s0.load(Wpk, ch + 2);
+            else
+                s1.load(Wpk, ch + 2);
+        } else if constexpr (NN > 0) {
+            if (ch + 2 == NCH)
+                n0.load(Wnext, 0);
+            else
+                n1.load(Wnext, 1);
+        }
         if (active) {
 #pragma unroll
             for (int ks = 0; ks < kBK; ks += 16) {
@@ -592,20 +608,19 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                                                             0, 0, 0);
             }
         }
-        // chunk ch + 1 (in set (ch + 1) % 2) into the other buffer, last read
-        // before the previous barrier
-        if (ch % 2 == 0)
-            r1.store(Wb, 1);
-        else
-            r0.store(Wb, 0);
+        // chunk ch + 1 into the other buffer, last read before the previous barrier
+        if (ch + 1 < NCH) {
+            if (ch % 2 == 0)
+                s1.store(Wb, 1);
+            else
+                s0.store(Wb, 0);
+        }
         __syncthreads();
     }
     // every wave is past its last read of Act and Wb
     if constexpr (NN > 0) {
-        Chunk<NN> first;
-        first.load(Wnext, 0);  // in flight during the epilogue
-        if constexpr (HIDDEN) first.store(Wb, 0);
-        else static_assert(HIDDEN, "only hidden layers are followed by a layer");
+        static_assert(HIDDEN, "only hidden layers are followed by a layer");
+        n0.store(Wb, 0);
     }
     if (active) {
 #pragma unroll
@@ -645,25 +660,30 @@ __global__ __launch_bounds__(512, 1) void k_ws_infer(
     __shared__ __attribute__((aligned(16))) bf16 Act[kIBM * LR];
     __shared__ __attribute__((aligned(16))) bf16 Wb[2 * D::H * kLdsRow];
     const int64_t c0 = (int64_t)blockIdx.x * kIBM;
+    Chunk<D::H> c1, n0a, n1a, n0b, n1b;
+    Chunk<D::No> n0c, n1c;
+    Chunk<32> d0, d1;  // (no layer after the output layer)
+    c1.load(W0, 1);    // layer 1's chunk 1, in flight through the copy and chunk 0
     {
         Chunk<D::H> first;
         first.load(W0, 0);  // layer 1's first weight chunk, in flight during the copy
         // the block's feature rows; rows past the last column repeat it
         // (their outputs are never stored)
-        constexpr int RU = D::Fp / 8;  // uint4 per row
+        constexpr int RU = D::Fp / 8;  // 16-byte pieces per row
         for (int q = threadIdx.x; q < kIBM * RU; q += 512) {
             const int lc = q / RU, k = q % RU;
             const int64_t c = c0 + lc < ncol ? c0 + lc : ncol - 1;
-            *(uint4*)(Act + lc * LR + 8 * k) = *(const uint4*)(X + c * D::Fp + 8 * k);
+            *(u32x4*)(Act + lc * LR + 8 * k) = *(const u32x4*)(X + c * D::Fp + 8 * k);
         }
         first.store(Wb, 0);
     }
     __syncthreads();
     const int rows = (int)min<int64_t>(kIBM, ncol - c0);
-    infer_layer<D::H, D::Fp, true, D::H>(Act, LR, Wb, W0, b0, nullptr, 0, W1);
-    infer_layer<D::H, D::H, true, D::H>(Act, LR, Wb, W1, b1, nullptr, 0, W2);
-    infer_layer<D::H, D::H, true, D::No>(Act, LR, Wb, W2, b2, nullptr, 0, W3);
-    infer_layer<D::No, D::H, false, 0>(Act, LR, Wb, W3, b3, Yg + c0 * D::No, rows, nullptr);
+    infer_layer<D::H, D::Fp, true, D::H>(Act, LR, Wb, W0, b0, nullptr, 0, W1, c1, n0a, n1a);
+    infer_layer<D::H, D::H, true, D::H>(Act, LR, Wb, W1, b1, nullptr, 0, W2, n1a, n0b, n1b);
+    infer_layer<D::H, D::H, true, D::No>(Act, LR, Wb, W2, b2, nullptr, 0, W3, n1b, n0c, n1c);
+    infer_layer<D::No, D::H, false, 0>(Act, LR, Wb, W3, b3, Yg + c0 * D::No, rows, nullptr, n1c,
+                                       d0, d1);
 }
 
 // ---- small training kernels ---------------------------------------------------
