@@ -10,7 +10,8 @@ and leaves ρe_int alone), up to what the Newton stopping rule leaves in the
 residual: at most the tolerance per cell and step, i.e. steps x depth x tol
 in water and that times the largest ρc_s in energy.  Plus the share of
 column-steps that converge, in the plain FP64, Newton-matrix and learned
-forms.
+forms, and the learned form's device time per step against the mixed
+form's.
 """
 import numpy as np
 import pytest
@@ -80,3 +81,38 @@ def test_full_workload_balances(gpu_lib, form):
     assert (np.abs(dE[conv]) <= bar_e[conv]).all()
     assert conv.mean() >= 0.8
     assert frac_conv >= (0.9 if form == "fp64" else 0.95)
+
+
+def test_learned_form_faster_than_mixed(gpu_lib):
+    """All 262,144 columns, 12 steps, each form's run captured once and then
+    timed as one graph replay: the learned form (warm start trained as
+    bench.py trains it, on the Newton matrix) takes less device time per step
+    than the mixed form (FP32 modified-Picard matrices) with at least 10%
+    fewer Newton iterations, the inference included."""
+    from clmsoil import _lib as L
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
+    from test_gpu_parity import _train_warm_start
+    b = inputs.make_batch("eh_262144_uneven", nsteps=NSTEPS)
+    w = b.workload
+    params, _ = _train_warm_start("eh_262144_uneven", 32, L.JACOBIAN_NEWTON)
+    out = {}
+    for form, jacobian, ws_params in (("mixed", L.JACOBIAN_PICARD, None),
+                                      ("learned", L.JACOBIAN_NEWTON, params)):
+        m = model_from_batch(b, jac_precision=32, solver=L.SOLVER_GROUP, jacobian=jacobian)
+        if ws_params is not None:
+            ws = NewtonWarmStart(m, seed=0)
+            ws.set_params(ws_params)
+            ws.enable(True)
+        sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                             tol=w.newton_tol)
+        sim.solve(b.top_water, b.top_heat, NSTEPS, w.steps_per_slot)  # capture
+        sim.reset(b.copy_state())
+        st, _ = sim.solve(None, None, NSTEPS, w.steps_per_slot, upload=False)
+        out[form] = (st.ms / NSTEPS, st.mean_iters)
+        m.close()
+    (ms_m, it_m), (ms_l, it_l) = out["mixed"], out["learned"]
+    print(f"mixed {ms_m:.3f} ms/step, {it_m:.2f} iterations; learned {ms_l:.3f} ms/step, "
+          f"{it_l:.2f} iterations ({100 * (1 - it_l / it_m):.0f}% fewer)", flush=True)
+    assert ms_l < ms_m
+    assert it_l <= 0.9 * it_m
