@@ -81,9 +81,14 @@ __global__ __launch_bounds__(256) void k_static_tails(bool eh, const int32_t* __
     o[6] = o[7] = (bf16)0.0f;
 }
 
-// lane j of column c's group writes entries j, 32 + j, ... of the column's
-// row: node j's state features from the two cells around it (each loaded
-// once), then the tail features
+// k_features gives each column 16 lanes: lane j0 forms nodes j0 and j0 + 16
+// (their loads independent, in flight together) and tail entries j0,
+// j0 + 16 of the column's row (half the waves of one lane per node, each
+// with twice the loads in flight)
+constexpr int kFeatGroup = 16;
+
+// lane j0 of column c's group: node j's state features from the two cells
+// around it (each loaded once), for j = j0, j0 + 16; then the tail entries
 __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict__ col_ptr,
                                              int64_t ncol, const double* __restrict__ par,
                                              const bf16* __restrict__ tails,
@@ -92,22 +97,26 @@ __device__ __forceinline__ void feature_lane(bool eh, const int32_t* __restrict_
                                              const double* __restrict__ xe,
                                              const double* __restrict__ ftw,
                                              const double* __restrict__ fth, double dt, int Fp,
-                                             int64_t c, int j, bf16* __restrict__ row) {
+                                             int64_t c, int j0, bf16* __restrict__ row) {
     const int64_t b = col_ptr[c];
     const int N = (int)(col_ptr[c + 1] - b);
     const double nu = par[P_NU * ncol + c], thr = par[P_THETA_R * ncol + c];
-    const NodeAtF a = node_at_f(j, N);
-    const double v0 = xv[b + a.i0], v1 = xv[b + a.i1];
-    row[j] = feat_v(a, (float)thr, (float)nu, (float)v0, (float)v1);
-    if (eh) {
-        const double rcd = par[P_RHOC_DS * ncol + c];
-        const double t0 = xi[b + a.i0], t1 = xi[b + a.i1];
-        row[kNodes + j] = feat_T(a, feat_dT(nu, rcd, v0, t0, xe[b + a.i0]),
-                                 feat_dT(nu, rcd, v1, t1, xe[b + a.i1]));
-        row[2 * kNodes + j] = feat_i(a, (float)nu, (float)t0, (float)t1);
+    const double rcd = eh ? par[P_RHOC_DS * ncol + c] : 0.0;
+#pragma unroll
+    for (int u = 0; u < kNodes / kFeatGroup; ++u) {
+        const int j = j0 + u * kFeatGroup;
+        const NodeAtF a = node_at_f(j, N);
+        const double v0 = xv[b + a.i0], v1 = xv[b + a.i1];
+        row[j] = feat_v(a, (float)thr, (float)nu, (float)v0, (float)v1);
+        if (eh) {
+            const double t0 = xi[b + a.i0], t1 = xi[b + a.i1];
+            row[kNodes + j] = feat_T(a, feat_dT(nu, rcd, v0, t0, xe[b + a.i0]),
+                                     feat_dT(nu, rcd, v1, t1, xe[b + a.i1]));
+            row[2 * kNodes + j] = feat_i(a, (float)nu, (float)t0, (float)t1);
+        }
     }
     const int S = state_features(eh);
-    for (int f = S + j; f < Fp; f += kGroup)
+    for (int f = S + j0; f < Fp; f += kFeatGroup)
         row[f] = tail_entry(eh, f - S, tails + c * kTails, ftw[c], eh ? fth[c] : 0.0, dt);
 }
 
@@ -117,10 +126,10 @@ __global__ __launch_bounds__(256) void k_features(
     const double* __restrict__ xe, const double* __restrict__ ftw, const double* __restrict__ fth,
     double dt, int Fp, bf16* __restrict__ X) {
     int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t c = gt / kGroup;
-    const int j = (int)(gt % kGroup);
+    int64_t c = gt / kFeatGroup;
+    const int j0 = (int)(gt % kFeatGroup);
     if (c >= ncol) return;
-    feature_lane(eh, col_ptr, ncol, par, tails, xv, xi, xe, ftw, fth, dt, Fp, c, j,
+    feature_lane(eh, col_ptr, ncol, par, tails, xv, xi, xe, ftw, fth, dt, Fp, c, j0,
                  X + c * (int64_t)Fp);
 }
 
@@ -745,8 +754,12 @@ __global__ __launch_bounds__(256) void k_mse(int M, int Nout, const float* __res
         l = d * d * inv;
         dY[i] = (bf16)(2.0f * d * inv);
     }
+    // one atomic per block (per wave, the adds on one address serialised)
+    __shared__ float part[4];
     for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off);
-    if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (part[0] + part[1]) + (part[2] + part[3]));
 }
 
 // rows of the training set gathered into a minibatch

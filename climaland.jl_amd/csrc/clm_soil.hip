@@ -989,9 +989,9 @@ int ws_batch_buffers(clm_ctx* c, WsState* w, int64_t M) {
 
 int ws_features_launch(clm_ctx* c, WsState* w, const double* ftw, const double* fth, double dt,
                        bf16* X) {
-    hipLaunchKernelGGL(ws::k_features, grid1(c->ncol * ws::kGroup), dim3(256), 0, c->stream, is_eh(c),
-                       c->d_col_ptr, c->ncol, c->d_cpar, w->tails, c->d_cv, c->d_ci, c->d_ce,
-                       ftw, fth ? fth : ftw, dt, w->d.Fp, X);
+    hipLaunchKernelGGL(ws::k_features, grid1(c->ncol * ws::kFeatGroup), dim3(256), 0, c->stream,
+                       is_eh(c), c->d_col_ptr, c->ncol, c->d_cpar, w->tails, c->d_cv, c->d_ci,
+                       c->d_ce, ftw, fth ? fth : ftw, dt, w->d.Fp, X);
     return CLM_OK;
 }
 
