@@ -768,7 +768,7 @@ __global__ __launch_bounds__(256, (kEhMinBlocks<JT, R>)) void k_eh_coop(
         }
         zc[k] = 0.5 * (z0 + z1);
         dzc[k] = z1 - z0;
-        EHAux a = eh_cell_aux(e, v, t, re);
+        EHAux a = eh_cell_aux(e, v, t, re, !warm);  // a warm start evaluates ψ at its x0
         double F = 0.0;
         if (on[k]) {
             double tau = 3.0 * heat_capacity(a.thl, t, e.rhoc_ds) * dzc[k] * dzc[k] / a.kap;

@@ -40,7 +40,10 @@ struct EHAux {
     double thl, kap, T, K, psi, psiw0;
 };
 
-__device__ __forceinline__ EHAux eh_cell_aux(const EHPar& e, double vl, double ti, double re) {
+// with_psi false: a.psi is not wanted (a caller that evaluates ψ at its own
+// iterate); a.psi is then left 0 where it would cost a separate evaluation
+__device__ __forceinline__ EHAux eh_cell_aux(const EHPar& e, double vl, double ti, double re,
+                                             bool with_psi = true) {
     EHAux a;
     a.thl = liq_fraction(e.v, vl, e.nu - ti);
     a.kap = thermal_cond(a.thl, ti, e.nu, e.om, e.qz, e.gr, e.k_dry, e.k_satu, e.k_satf,
@@ -59,7 +62,7 @@ __device__ __forceinline__ EHAux eh_cell_aux(const EHPar& e, double vl, double t
         a.psiw0 = p.S < 1.0 ? p.psi : -0.0;
     } else {
         a.K = impvisc * hyd_cond(e.v, eff_sat(e.v, e.nu, vl));
-        a.psi = psi_dpsi(e.v, e.nu - ti, vl).psi;
+        a.psi = with_psi ? psi_dpsi(e.v, e.nu - ti, vl).psi : 0.0;
         a.psiw0 = matric_total(e.v, e.nu, a.thl, ti);
     }
     return a;
