@@ -386,13 +386,16 @@ int clm_ws_forward(clm_ctx* ctx, const float* X, int64_t M, float* Y);
 int clm_ws_gradient(clm_ctx* ctx, const float* X, const float* T, const float* mask, int64_t M,
                     float* loss, float* grad);
 /* The warm start's guess x0 - U0 for the device state and the clm_set_forcing
- * fluxes, per cell in CSR order: ϑ_l and (EH) ρe_int.  fused = 1 runs the one-
- * kernel inference that clm_step / clm_run use; fused = 0 the features, layer-
- * by-layer GEMMs and guess kernels (the same arithmetic, for checking). */
+ * fluxes, per cell in CSR order: ϑ_l and (EH) ρe_int.  fused = 1 runs the
+ * inference that clm_step / clm_run use (the feature kernel, then the four
+ * layers in one kernel); fused = 0 the features, layer-by-layer GEMMs and
+ * guess kernels (the same arithmetic, for checking). */
 int clm_ws_guess(clm_ctx* ctx, double dt, int32_t fused, double* guess_vartheta_l,
                  double* guess_rho_e_int);
-/* Device time of one warm-start inference (features, network, guess) on the
- * current state and clm_set_forcing fluxes, averaged over reps (HIP events). */
+/* Device time of one warm-start inference as a step runs it (the feature
+ * rows, then the network's output rows; the step kernels form the per-cell
+ * guesses from those rows) on the current state and clm_set_forcing fluxes,
+ * averaged over reps (HIP events). */
 int clm_ws_time(clm_ctx* ctx, double dt, int32_t reps, double* ms_per_call);
 
 #ifdef __cplusplus
