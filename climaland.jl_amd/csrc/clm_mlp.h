@@ -540,10 +540,10 @@ struct Chunk {
 
 // one layer over the block's 128 rows: Act[:, 0:K] . W^T + bias; hidden layers
 // write GELU(.) as BF16 into Act[:, 0:N], the output layer writes FP32 rows
-// [rows][N] to Y in global memory (rows past `rows` are not stored).  On entry chunk 0
-// of W is in Wb buffer 0 (behind a barrier); on exit, if NN > 0, chunk 0 of
-// the next layer's weights Wnext ([NN][K'] chunk-major) is, its load having
-// overlapped this layer's epilogue.
+// [rows][N] to Y in global memory (rows past `rows` are not stored).  On entry
+// chunk 0 of W is in Wb buffer 0 (behind a barrier) and chunk 1 in s1; on
+// exit, if NN > 0, chunk 0 of the next layer's weights Wnext ([NN][K']
+// chunk-major) is in buffer 0 and its chunk 1 in n1.
 template <int N, int K, bool HIDDEN, int NN>
 __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
                                             bf16* __restrict__ Wb,
@@ -569,11 +569,6 @@ __device__ __forceinline__ void infer_layer(bf16* __restrict__ Act, int lds_row,
         for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-    // Chunk c of W is loaded into register set c % 2 two chunks ahead of its
-    // use (two chunks of MFMAs for the load to arrive in) and stored into LDS
-    // buffer c % 2 behind the barrier that ends the reads of that buffer.
-    // Loads and stores are unconditional (past the last chunk they repeat
-    // it into the free buffer), so the sets stay in registers.
     // Chunk c of W sits in register set c % 2 (s0 even, s1 odd; s1 arrives
     // holding chunk 1) from two chunks before its use, so its load has two
     // chunks of MFMAs to arrive in, and goes into LDS buffer c % 2 behind
