@@ -111,6 +111,13 @@ struct clm_ctx {
     // stream, graph, events
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // the depth bins of a step are independent: each launches on a stream of
+    // its own (forked from and joined back into `stream`, also under graph
+    // capture), so one bin's last waves share the GPU with the next bin's
+    // first instead of draining alone
+    static constexpr int kBinStreams = 4;
+    hipStream_t bin_stream[kBinStreams] = {};
+    hipEvent_t bin_fork = nullptr, bin_join[kBinStreams] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -653,12 +660,26 @@ template <typename JT>
 void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
                     int32_t* iters, const StepExtra& x) {
     const bool pair = pairing_applies(c);
+    const hipStream_t main = c->stream;
+    (void)hipEventRecord(c->bin_fork, main);
+    int k = 0;  // bins launched
     for (int b = 0; b < (int)c->bins.size(); ++b) {
         if (pair && b == kPairShallowBin) continue;
+        if (c->bins[b].n == 0) continue;
+        const int si = k % clm_ctx::kBinStreams;
+        // a stream reused within the step waits for its previous bin anyway
+        if (k < clm_ctx::kBinStreams) (void)hipStreamWaitEvent(c->bin_stream[si], c->bin_fork, 0);
+        c->stream = c->bin_stream[si];
         if (pair && b == kPairDeepBin)
             launch_pair<JT>(c, ftw, fth, dt, iters, x);
         else
             launch_bin_any<JT>(c, c->bins[b], ftw, fth, dt, iters, x);
+        c->stream = main;
+        ++k;
+    }
+    for (int i = 0; i < std::min(k, clm_ctx::kBinStreams); ++i) {
+        (void)hipEventRecord(c->bin_join[i], c->bin_stream[i]);
+        (void)hipStreamWaitEvent(main, c->bin_join[i], 0);
     }
 }
 
@@ -1401,9 +1422,17 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
             return fail(CLM_E_HIP);
         }
     }
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->bin_fork, hipEventDisableTiming) != hipSuccess) {
         set_err(c, CLM_E_HIP, "hipEventCreate failed");
         return fail(CLM_E_HIP);
+    }
+    for (int i = 0; i < clm_ctx::kBinStreams; ++i) {
+        if (hipStreamCreateWithFlags(&c->bin_stream[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c->bin_join[i], hipEventDisableTiming) != hipSuccess) {
+            set_err(c, CLM_E_HIP, "bin streams: create failed");
+            return fail(CLM_E_HIP);
+        }
     }
     {
         int r = scratch_address(c, &c->d_scratch);
@@ -1490,6 +1519,14 @@ int clm_destroy(clm_ctx* c) {
     c->ws = nullptr;
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->bin_fork) hipEventDestroy(c->bin_fork);
+    for (int i = 0; i < clm_ctx::kBinStreams; ++i) {
+        if (c->bin_stream[i]) {
+            hipStreamSynchronize(c->bin_stream[i]);
+            hipStreamDestroy(c->bin_stream[i]);
+        }
+        if (c->bin_join[i]) hipEventDestroy(c->bin_join[i]);
+    }
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
     return CLM_OK;

@@ -37,7 +37,10 @@
  *     never aborts; clm_last_error(ctx) gives the message.
  *   - A context is used from one host thread at a time; its work is
  *     ordered on its own HIP stream.  Calls that return data to the host
- *     synchronise that stream.
+ *     synchronise that stream.  Within a step the independent depth bins
+ *     run on four internal streams that fork from that stream and join back
+ *     into it (with events, so this also holds under graph capture): work
+ *     queued on the ctx stream after a call sees all of the call's results.
  */
 #ifndef CLM_SOIL_H
 #define CLM_SOIL_H
