@@ -6,8 +6,9 @@ Bars:
   * network forward / loss / gradient vs the CPU restatement with the same
     BF16 rounding points: relative L2 error <= 1e-2 per output and per
     layer gradient (rounding of intermediates may flip on near-ties);
-  * warm start with a zero guess: the oracle's warm-started step (x0 = U0)
-    to 1e-10 on well-conditioned columns;
+  * warm start with a zero guess (x0 = U0), and with the nonzero guesses of
+    perturbed parameters: the oracle's warm-started step from the same
+    guesses to 1e-10 on well-conditioned columns;
   * trained warm start: same root as plain Newton to the Newton tolerance,
     and it reports the iterations it saves.
 """
@@ -150,10 +151,16 @@ def _one_step(b, sim, m):
     return sim.state(), it[0]
 
 
+@pytest.mark.parametrize("params", ["zero", "perturbed"])
 @pytest.mark.parametrize("name", ["rre_131072_uneven", "eh_262144_uneven"])
-def test_zero_guess_matches_oracle_warm_start(gpu_lib, name):
-    """With all network parameters zero the guess is exactly 0: the fused
-    kernels' warm-start path must equal the oracle's orc_step_guess."""
+def test_zero_guess_matches_oracle_warm_start(gpu_lib, name, params):
+    """The step kernels' warm-start path against the oracle's orc_step_guess
+    given the same guesses.  With all network parameters zero the guess is
+    exactly 0.  With perturbed parameters the guesses are nonzero and differ
+    by cell; the oracle gets those of clm_ws_guess (k_guess over the network
+    rows), while the step kernels form theirs from the same rows themselves:
+    a guess formed differently, or a row missed or stale, would change the
+    iterations."""
     from clmsoil import inputs
     from clmsoil.model import LandSimulation, NewtonWarmStart
     from oracle.oracle import OracleModel
@@ -161,19 +168,30 @@ def test_zero_guess_matches_oracle_warm_start(gpu_lib, name):
     w = b.workload
     m = _model(b)
     ws = NewtonWarmStart(m, seed=3)
-    ws.set_params(np.zeros(ws.n_params, dtype=np.float32))
+    if params == "zero":
+        ws.set_params(np.zeros(ws.n_params, dtype=np.float32))
+    else:
+        rng = np.random.default_rng(5)
+        ws.set_params((ws.params() + 0.01 * rng.standard_normal(ws.n_params)).astype(np.float32))
     ws.enable(True)
     sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters, tol=w.newton_tol)
+    if params == "zero":
+        gv = np.zeros(b.ncell)
+        ge = gv if b.top_heat is not None else None
+    else:
+        _set_forcing(m, b)  # the forcing of the step's slot 0
+        sim._push()
+        gv, ge = ws.guess(w.dt, fused=True)
+        assert np.abs(gv).max() > 0 and np.unique(gv).size > b.ncol
     Yg, itg = _one_step(b, sim, m)
     fw = b.top_water[0].copy()
     fh = None if b.top_heat is None else b.top_heat[0].copy()
-    zero = np.zeros(b.ncell)
     outs = {}
     for v in ("", "xprec", "fma"):
         o = OracleModel.from_batch(b, variant=v)
         Y = b.copy_state()
-        it, _ = o.step(Y, w.dt, w.newton_max_iters, w.newton_tol, fw, fh, guess_v=zero,
-                       guess_e=zero if b.top_heat is not None else None)
+        it, _ = o.step(Y, w.dt, w.newton_max_iters, w.newton_tol, fw, fh, guess_v=gv,
+                       guess_e=ge)
         outs[v] = (Y, it)
     Yo, ito = outs[""]
     ok = column_ok(b, Yo) & column_ok(b, Yg) & (ito < w.newton_max_iters)
@@ -194,7 +212,7 @@ def test_zero_guess_matches_oracle_warm_start(gpu_lib, name):
     for k in keys:
         fl = 1.0 if k == "rho_e_int" else 1e-12
         e = np.where(np.repeat(well, b.nlayers), rel_err(Yg[k], Yo[k], fl), 0.0)
-        print(f"{name} {k}: zero-guess warm start max rel err {e.max():.2e} on {well.sum()} "
+        print(f"{name} {k}: {params} warm start max rel err {e.max():.2e} on {well.sum()} "
               f"columns", flush=True)
         assert e.max() <= 1e-10
     m.close()
