@@ -39,22 +39,23 @@ HBM_PEAK_GBS = 8000.0
 # plain device copy of known size reads for it, WRITE likewise); raw sums
 # beside them.  Figures, date, build and commands:
 # profiles/k_eh_coop_eh_262144_uneven.md.
-MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 2824.5e6, "raw": 2002.5e6,
+MEASURED_TRAFFIC = {("eh_262144_uneven", "learned"): {"scaled": 2836.4e6, "raw": 2008.4e6,
                                                       "source": "rocprofv3 2026-12-03, build "
-                                                                "a8b1fda, 3 early steps"},
-                    ("eh_262144_uneven", "newton"): {"scaled": 2507.6e6, "raw": 1844.1e6,
+                                                                "b795bf6, 3 early steps"},
+                    ("eh_262144_uneven", "newton"): {"scaled": 2507.1e6, "raw": 1843.9e6,
                                                      "source": "rocprofv3 2026-12-03, build "
-                                                               "a8b1fda, 3 early steps"}}
+                                                               "b795bf6, 3 early steps"}}
 
 # What bounds the fused step: VALU issue.  rocprofv3 counters of the step
 # kernels on 3 early steps of eh_262144_uneven, learned form
 # (profiles/k_eh_coop_eh_262144_uneven.md): SQ_INSTS_VALU_FLOPS_FP64 x 64
-# lanes per step, SQ_INSTS_VALU per step, and the kernels' time per step of
-# the same workload in a kernel trace.
-MEASURED_VALU = {"eh_262144_uneven": {"fp64_flop_per_step": 2.0403e10,
-                                      "valu_instructions_per_step": 4.769e8,
-                                      "kernel_ms_per_step": 1.3718,
-                                      "source": "rocprofv3 2026-12-03, build a8b1fda, 3 early "
+# lanes per step, SQ_INSTS_VALU per step, and the time per step of the same
+# workload's bin launches in a kernel trace (first start to last end: the
+# bins of a step run concurrently).
+MEASURED_VALU = {"eh_262144_uneven": {"fp64_flop_per_step": 1.9933e10,
+                                      "valu_instructions_per_step": 4.685e8,
+                                      "kernel_ms_per_step": 1.2798,
+                                      "source": "rocprofv3 2026-12-03, build b795bf6, 3 early "
                                                 "steps, learned form"}}
 FP64_VALU_PEAK_TFLOPS = 78.6   # MI355X FP64 vector
 CU_COUNT, ENGINE_GHZ = 256, 2.4
