@@ -271,3 +271,51 @@ def test_trained_warm_start_saves_iterations(gpu_lib, name, jac):
     # from a root) falls back to U0, so no column lands far from plain Newton's
     assert dcol.max() <= 1e-4
     assert kw.mean() <= kp.mean()
+
+
+def test_run_with_warm_start_equals_steps(gpu_lib):
+    """An eight-step run captured as one graph against the same eight steps
+    one clm_step call at a time: the states are equal bit for bit and the
+    Newton iterations agree.  The learned form on the Newton matrix, paired
+    waves active, a perturbed network so the guesses differ by column, and
+    the forcing slot changing inside the run (six steps per slot), so each
+    step's feature rows must use that step's state and forcing."""
+    from clmsoil import _lib as L
+    from clmsoil import inputs
+    from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
+    nsteps = 8
+    b = inputs.make_batch("eh_262144_uneven", ncol=3000, nsteps=nsteps)
+    w = b.workload
+    assert len({b.forcing_slot(s) for s in range(nsteps)}) > 1
+    params = None
+    outs = []
+    for mode in ("run", "steps"):
+        m = model_from_batch(b, jac_precision=32, solver=L.SOLVER_GROUP,
+                             jacobian=L.JACOBIAN_NEWTON)
+        for mx, g, r in ((16, 16, 1), (24, 32, 1), (32, 32, 1), (40, 64, 1)):
+            m.set_shape(mx, g, r)
+        assert m.pairing_active()
+        ws = NewtonWarmStart(m, seed=3)
+        if params is None:
+            rng = np.random.default_rng(7)
+            params = (ws.params() + 0.02 * rng.standard_normal(ws.n_params)).astype(np.float32)
+        ws.set_params(params)
+        ws.enable(True)
+        sim = LandSimulation(m, b.copy_state(), w.dt, max_iters=w.newton_max_iters,
+                             tol=w.newton_tol)
+        if mode == "run":
+            st, _ = sim.solve(b.top_water, b.top_heat, nsteps, w.steps_per_slot)
+            iters = st.mean_iters * nsteps
+        else:
+            iters = 0.0
+            for s in range(nsteps):
+                _set_forcing(m, b, b.forcing_slot(s))
+                iters += sim.step().mean_iters
+        outs.append((sim.state(), iters))
+        m.close()
+    (Yr, ir), (Ys, is_) = outs
+    for k in Yr:
+        assert np.array_equal(Yr[k], Ys[k], equal_nan=True), k
+    print(f"warm-started run of {nsteps} steps equals the steps one call at a time; Newton "
+          f"iterations per column {ir:.3f} / {is_:.3f}", flush=True)
+    assert abs(ir - is_) <= 1e-9 * max(1.0, ir)
