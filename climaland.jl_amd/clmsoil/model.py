@@ -197,7 +197,10 @@ class _SoilModel:
     def __init__(self, *, parameters, domain: ColumnBatchDomain, boundary_conditions: dict,
                  sources: tuple = (), jac_precision: int = 64, newton_max_iters: int = 3,
                  newton_tol: float = 0.0, solver: int = L.SOLVER_AUTO,
-                 jacobian: int = L.JACOBIAN_PICARD):
+                 jacobian: int = L.JACOBIAN_PICARD, stream: int | None = None):
+        """stream: a HIP stream handle (an integer, e.g. torch's
+        `Stream.cuda_stream`) for the context's work, which the caller keeps
+        alive until close(); None: the library's own stream."""
         self.parameters = parameters
         self.domain = domain
         self.boundary_conditions = boundary_conditions
@@ -222,7 +225,8 @@ class _SoilModel:
                        L.iptr(self._col_ptr), L.dptr(self._z_face), L.dptr(self._params),
                        self._params.shape[0], jac_precision, newton_max_iters, newton_tol,
                        solver, self._top_kind(), jacobian, 0)
-        L.check(self._lib, None, self._lib.clm_create(C.byref(self._ctx), C.byref(cfg), None))
+        L.check(self._lib, None, self._lib.clm_create(C.byref(self._ctx), C.byref(cfg),
+                                                      C.c_void_p(stream) if stream else None))
         self.jac_precision = jac_precision
         self.jacobian = jacobian
         self.set_forcing()
@@ -791,9 +795,10 @@ def _column_norm(m: _SoilModel, dx: dict, x: dict) -> np.ndarray:
 
 def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | None = None,
                      tol: float | None = None, slot: int = 0, solver: int = L.SOLVER_AUTO,
-                     jacobian: int = L.JACOBIAN_PICARD):
+                     jacobian: int = L.JACOBIAN_PICARD, stream: int | None = None):
     """Build the model a synthetic ColumnBatch describes.  jacobian: the
-    Newton matrix of the EnergyHydrology group step (L.JACOBIAN_*)."""
+    Newton matrix of the EnergyHydrology group step (L.JACOBIAN_*); stream:
+    a HIP stream handle for the context (None: the library's own)."""
     w = batch.workload
     P = batch.params
     hcm = vanGenuchten(P[L.P_VG_ALPHA], P[L.P_VG_N])
@@ -803,6 +808,8 @@ def model_from_batch(batch, jac_precision: int | None = None, max_iters: int | N
               newton_tol=w.newton_tol if tol is None else tol, solver=solver)
     if jacobian != L.JACOBIAN_PICARD:
         kw["jacobian"] = jacobian
+    if stream:
+        kw["stream"] = stream
     if batch.model == L.MODEL_RICHARDS:
         par = RichardsParameters(P[L.P_NU], hcm, P[L.P_KSAT], P[L.P_SS], P[L.P_THETA_R])
         tw = batch.top_water[slot]

@@ -32,10 +32,10 @@ def _subset(b, deep_share):
     return b.column_subset(cols), len(deep), len(shallow)
 
 
-def _run(b, jac, jacobian, pairing, ws_params=None, nsteps=6):
+def _run(b, jac, jacobian, pairing, ws_params=None, nsteps=6, stream=None):
     from clmsoil.model import LandSimulation, NewtonWarmStart, model_from_batch
     w = b.workload
-    m = model_from_batch(b, jac_precision=jac, solver=1, jacobian=jacobian)
+    m = model_from_batch(b, jac_precision=jac, solver=1, jacobian=jacobian, stream=stream)
     for mx, g, r in SHAPES:
         m.set_shape(mx, g, r)
     m.set_pairing(pairing)
@@ -77,3 +77,40 @@ def test_paired_waves_equal_separate_launches(gpu_lib, form, deep_share):
     print(f"{form}: {b.ncol} columns ({nd} of 33-40 layers, {ns} of 17-24): paired run equals "
           f"separate launches bit for bit; mean Newton iterations {stp.mean_iters:.2f}, "
           f"{stp.ms / 6:.3f} vs {sts.ms / 6:.3f} ms/step", flush=True)
+
+
+def test_callers_stream_equals_own_stream(gpu_lib):
+    """A context created on a caller's HIP stream against one on the
+    library's own stream: a step's depth bins fork from and join back into
+    whichever stream the context has, in the captured run, so the states and
+    Newton counts are equal bit for bit.  Learned form on the Newton matrix
+    with paired waves and a perturbed network.  The caller's stream comes
+    from the HIP runtime the library itself resolved (non-blocking, default
+    priority, as torch creates its streams)."""
+    import ctypes as C
+    from clmsoil import _lib as L
+    from clmsoil import inputs
+    from clmsoil.model import NewtonWarmStart, model_from_batch
+    b0 = inputs.make_batch("eh_262144_uneven", ncol=6000, nsteps=8)
+    b, _, _ = _subset(b0, 1.0)
+    m = model_from_batch(b, jac_precision=32, solver=1, jacobian=L.JACOBIAN_NEWTON)
+    ws = NewtonWarmStart(m, seed=3)
+    rng = np.random.default_rng(13)
+    params = (ws.params() + 0.02 * rng.standard_normal(ws.n_params)).astype(np.float32)
+    m.close()
+    hip = C.CDLL("libamdhip64.so.7")  # the copy already in the process, by its soname
+    hip.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    s = C.c_void_p()
+    assert hip.hipStreamCreateWithFlags(C.byref(s), 1) == 0 and s.value  # non-blocking
+    try:
+        Yc, itc, stc, act_c = _run(b, 32, L.JACOBIAN_NEWTON, True, params, 8, s.value)
+    finally:
+        hip.hipStreamDestroy(s)
+    Yo, ito, sto, act_o = _run(b, 32, L.JACOBIAN_NEWTON, True, params, 8)
+    assert act_c and act_o
+    for k in Yc:
+        assert np.array_equal(Yc[k], Yo[k], equal_nan=True), k
+    assert np.array_equal(itc, ito)
+    print(f"{b.ncol} columns, 8 steps on the caller's stream equal the library's own stream bit "
+          f"for bit; mean Newton iterations {stc.mean_iters:.2f}", flush=True)
