@@ -118,6 +118,10 @@ struct clm_ctx {
     static constexpr int kBinStreams = 4;
     hipStream_t bin_stream[kBinStreams] = {};
     hipEvent_t bin_fork = nullptr, bin_join[kBinStreams] = {};
+    // inside a run, a step's regrouping runs on bin_stream[0] beside the next
+    // step's warm-start inference; the next step's fork waits for rg_done
+    hipEvent_t rg_start = nullptr, rg_done = nullptr;
+    bool rg_pending = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
@@ -661,6 +665,10 @@ void launch_group_t(clm_ctx* c, const double* ftw, const double* fth, double dt,
                     int32_t* iters, const StepExtra& x) {
     const bool pair = pairing_applies(c);
     const hipStream_t main = c->stream;
+    if (c->rg_pending) {  // the last step's regrouping, before any bin reads its list
+        (void)hipStreamWaitEvent(main, c->rg_done, 0);
+        c->rg_pending = false;
+    }
     (void)hipEventRecord(c->bin_fork, main);
     int k = 0;  // bins launched
     for (int b = 0; b < (int)c->bins.size(); ++b) {
@@ -783,8 +791,12 @@ void launch_regroup(clm_ctx* c, const int32_t* iters) {
 void ws_launch_guess(clm_ctx* c, const double* ftw, const double* fth, double dt,
                      StepExtra* x);
 
+// defer_regroup (a run's steps but its last): the regrouping goes on
+// bin_stream[0] and the next step's fork waits for it, so it runs beside
+// that step's warm-start inference on the ctx stream
 void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, int32_t* iters,
-                 double* u0out = nullptr, bool allow_warm = true, const Surface* sf = nullptr) {
+                 double* u0out = nullptr, bool allow_warm = true, const Surface* sf = nullptr,
+                 bool defer_regroup = false) {
     if (is_group(c)) {
         StepExtra x = current_extra(c);
         if (sf) x.sf = *sf;
@@ -794,7 +806,18 @@ void launch_step(clm_ctx* c, const double* ftw, const double* fth, double dt, in
             launch_group_t<float>(c, ftw, fth, dt, iters, x);
         else
             launch_group_t<double>(c, ftw, fth, dt, iters, x);
-        launch_regroup(c, iters);
+        if (defer_regroup) {
+            const hipStream_t main = c->stream;
+            (void)hipEventRecord(c->rg_start, main);
+            (void)hipStreamWaitEvent(c->bin_stream[0], c->rg_start, 0);
+            c->stream = c->bin_stream[0];
+            launch_regroup(c, iters);
+            c->stream = main;
+            (void)hipEventRecord(c->rg_done, c->bin_stream[0]);
+            c->rg_pending = true;
+        } else {
+            launch_regroup(c, iters);
+        }
         return;
     }
     if (c->jac_prec == 32)
@@ -1423,7 +1446,9 @@ int clm_create(clm_ctx** out, const clm_config* cfg, void* hip_stream) {
         }
     }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&c->bin_fork, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->bin_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->rg_start, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->rg_done, hipEventDisableTiming) != hipSuccess) {
         set_err(c, CLM_E_HIP, "hipEventCreate failed");
         return fail(CLM_E_HIP);
     }
@@ -1520,6 +1545,8 @@ int clm_destroy(clm_ctx* c) {
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->bin_fork) hipEventDestroy(c->bin_fork);
+    if (c->rg_start) hipEventDestroy(c->rg_start);
+    if (c->rg_done) hipEventDestroy(c->rg_done);
     for (int i = 0; i < clm_ctx::kBinStreams; ++i) {
         if (c->bin_stream[i]) {
             hipStreamSynchronize(c->bin_stream[i]);
@@ -1989,6 +2016,7 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
     if (!c->gexec || c->g_nsteps != nsteps || c->g_sps != steps_per_slot ||
         c->g_tab != nslots || c->g_dt != dt) {
         destroy_graph(c);
+        c->rg_pending = false;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int32_t s = 0; s < nsteps; ++s) {
             int64_t slot = (s / steps_per_slot) % nslots;
@@ -1996,12 +2024,14 @@ int clm_run(clm_ctx* c, double t0, double dt, int32_t nsteps, int32_t steps_per_
             if (atm) {
                 Surface sf = surface_of(c->d_tab_sfc, slot * c->ncol);
                 launch_step(c, nullptr, nullptr, dt, c->d_iters + (int64_t)s * nv, nullptr,
-                            false, &sf);
+                            false, &sf, s + 1 < nsteps);
             } else {
                 launch_step(c, c->d_tab_w + slot * nv, c->d_tab_h + slot * nv, dt,
-                            c->d_iters + (int64_t)s * nv);
+                            c->d_iters + (int64_t)s * nv, nullptr, true, nullptr,
+                            s + 1 < nsteps);
             }
         }
+        c->rg_pending = false;  // the last step regroups on the ctx stream
         hipError_t e = hipStreamEndCapture(c->stream, &c->graph);
         if (e != hipSuccess) {
             c->graph = nullptr;
