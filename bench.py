@@ -339,7 +339,10 @@ def main():
         "warm_start_inference": None if not learned else {
             "ms_per_step": r["infer_ms"],
             "gemm_tflops": mlp_flops(batch_steps) / (r["infer_ms"] * 1e-3) / 1e12,
-            "mfma_bf16_peak_tflops": 2500.0},
+            "mfma_bf16_peak_tflops": 2500.0,
+            "note": "clm_ws_time: k_features then k_ws_infer, HIP events on the context's "
+                    "stream; gemm_tflops = the four layers' FLOPs over both kernels' time "
+                    "(k_ws_infer alone: profiles/k_ws_infer_eh_262144_uneven.md)"},
         "newton": {"mean_iters": st.mean_iters, "max_iters": st.max_iters_used,
                    "nonconverged_column_steps": st.n_nonconverged,
                    "nonfinite_column_steps": st.n_nan,
